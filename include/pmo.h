@@ -1,0 +1,165 @@
+/*
+ * pmo.h -- C ABI of libpmo.so, the MI355X (gfx950) voxel Monte Carlo photon walk.
+ *
+ * The library replaces the walk kernels of pyMonteOpt's voxel models:
+ *   - the Numba CPU walk  vmc_kernel(add, p, v, w, ma, ms, n, g, voxel_model, l, nPh, end_point)
+ *     (pymopt/voxel/_kernel.py:23-172), and
+ *   - the CuPy RawKernel  cuVMC(add, p, v, w, ma, ms, n, g, voxel_model, l, M, L, nPh,
+ *     end_point, rand_seed)  (pymopt/voxel_gpu/_cukernel.py:27-227), launched from
+ *     BaseVoxelMonteCarlo.start (pymopt/voxel_gpu/_classes.py:75-129).
+ *
+ * Differences in shape from those interfaces, all deliberate:
+ *   - packets are generated on the device from the source description and their id
+ *     (no 28 B/packet host state, no H2D/D2H copies of add/p/v/w);
+ *   - the walk accumulates FP64 tallies on the device (Rd, Tt, absorbed weight per voxel,
+ *     radial/angular/line/annular exit tallies, an exact energy ledger, event counts);
+ *   - the per-packet exit state (the reference's add/p/v/w after the walk) is available
+ *     as an optional dump for small runs;
+ *   - every call is asynchronous on the caller's HIP stream and returns an error code.
+ *
+ * Conventions: all pointers marked "device" are HIP device pointers owned by the caller;
+ * the library never frees them and keeps no device state between calls except its code
+ * objects. Tally buffers are accumulated into (caller zeroes them). The library allocates
+ * nothing: per-call scratch lives in the caller-provided workspace of
+ * pmo_workspace_bytes() bytes, so every entry point may be captured into a HIP graph.
+ * A `stream` argument is a hipStream_t (NULL = the default stream).
+ * Return value: PMO_OK or a negative PMO_ERR_* code; pmo_last_error() gives the message
+ * (thread-local).
+ */
+#ifndef PMO_H
+#define PMO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PMO_ABI_VERSION 1
+
+/* ---- error codes ---------------------------------------------------------------- */
+#define PMO_OK 0
+#define PMO_ERR_INVALID (-1)     /* bad argument: null pointer, size or range        */
+#define PMO_ERR_LABEL (-2)       /* label volume / optics table inconsistency         */
+#define PMO_ERR_HIP (-3)         /* HIP runtime error (launch, device query)          */
+#define PMO_ERR_UNSUPPORTED (-4) /* request outside what this build implements        */
+
+/* ---- beam kinds ------------------------------------------------------------------ */
+#define PMO_BEAM_PENCIL 0 /* all packets start at the centre of the launch voxel's top face  */
+#define PMO_BEAM_FLAT 1   /* (x, y) uniform in a disc of radius `radius` (mm)               */
+#define PMO_BEAM_GAUSS 2  /* TEM00: x, y ~ N(0, radius/2), radius = beam waist w (mm)       */
+
+/* ---- run flags ------------------------------------------------------------------- */
+#define PMO_F_FP64 (1u << 0)           /* walk in double precision (checking build)        */
+#define PMO_F_NORMAL_FRESNEL (1u << 1) /* at exactly normal incidence use the Fresnel limit
+                                          ((ni-nt)/(ni+nt))^2; default keeps R = 0 there,
+                                          as pymopt/voxel/_kernel.py:89-93 does            */
+
+/* ---- scalar tally slots (double) ---------------------------------------------------- */
+enum {
+    PMO_S_RD = 0,        /* weight escaping with v_z < 0 (reference getRdTtRate Rd)           */
+    PMO_S_TT = 1,        /* weight escaping with v_z > 0 (reference Tt)                       */
+    PMO_S_OTHER = 2,     /* weight escaping with v_z == 0                                     */
+    PMO_S_ABS = 3,       /* absorbed weight (sum of all deposits)                             */
+    PMO_S_KILL = 4,      /* weight removed by Russian roulette                                */
+    PMO_S_GAIN = 5,      /* weight added by roulette survival (m-1)*w                         */
+    PMO_S_W0 = 6,        /* launched weight                                                   */
+    PMO_S_FACE = 7,      /* 6 slots: escaped weight through x-, x+, y-, y+, z- (top), z+      */
+    PMO_S_COUNT = 13
+};
+
+/* ---- event counters (uint64) -------------------------------------------------------- */
+enum {
+    PMO_EV_PACKETS = 0,  /* packets launched                                                  */
+    PMO_EV_INTERACT = 1, /* hops that ended inside the medium (drop + roulette test + spin)   */
+    PMO_EV_CROSS = 2,    /* voxel-face crossings (including reflections)                      */
+    PMO_EV_FRESNEL = 3,  /* crossings with a refractive-index change (Fresnel draw)           */
+    PMO_EV_ROULETTE = 4, /* roulette tests (w <= wth)                                         */
+    PMO_EV_KILLED = 5,   /* packets ended by roulette                                         */
+    PMO_EV_ESCAPED = 6,  /* packets that entered an end (ambient) voxel                       */
+    PMO_EV_COUNT = 7
+};
+
+/* ---- geometry store ----------------------------------------------------------------- */
+typedef struct pmo_volume {
+    const uint8_t* labels; /* device, C order [nx][ny][nz] (z fastest); the 1-voxel outer shell
+                              must hold the end label n_labels                                */
+    int32_t nx, ny, nz;    /* padded extents (reference voxel_model.shape)                     */
+    int32_t n_labels;      /* tissue labels 0..n_labels-1; label n_labels is the end label     */
+    float pitch;           /* voxel edge (mm), the reference's voxel_space / l                 */
+    const float* optics;   /* device, float4 per label, n_labels+1 entries: (mu_a, mu_s, g, n);
+                              entry n_labels holds n_ambient (mu_a, mu_s, g unused)            */
+} pmo_volume;
+
+typedef struct pmo_source {
+    int32_t kind;          /* PMO_BEAM_*                                                       */
+    int32_t ix, iy, iz;    /* launch voxel = beam centre (reference _set_inital_add)           */
+    float radius;          /* disc radius / Gaussian waist (mm), unused for the pencil beam    */
+    float dir[3];          /* launch direction, unit vector (reference _set_inital_vector)     */
+    float w0;              /* launch weight, 1 - Rsp (reference _set_inital_w)                 */
+} pmo_source;
+
+typedef struct pmo_run {
+    uint64_t n_packets;     /* packets in this call                                            */
+    uint64_t first_packet;  /* packet ids are [first_packet, first_packet + n_packets)         */
+    uint32_t seed;          /* Philox key word 0 (the reference's rand_seed)                   */
+    uint32_t stream;        /* Philox key word 1 (independent streams for one seed)            */
+    float wth;              /* roulette threshold (reference 1e-4)                             */
+    float roulette_m;       /* roulette survival 1/m, weight multiplier m (reference 10)       */
+    uint32_t flags;         /* PMO_F_*                                                         */
+    int32_t block_size;     /* threads per block, multiple of 64, <= 1024; 0 = default         */
+    int32_t grid_blocks;    /* persistent blocks; 0 = fill the device at full occupancy        */
+    int32_t packets_per_lane; /* a wave claims 64*this packet ids per work-counter atomic; 0 = default */
+} pmo_run;
+
+typedef struct pmo_tallies {
+    double* scalars;        /* device [PMO_S_COUNT], required                                  */
+    uint64_t* events;       /* device [PMO_EV_COUNT] or NULL                                   */
+    double* absorbed;       /* device [nx*ny*nz] deposited weight per voxel (padded index) or NULL */
+    int32_t nr;             /* radial bins (i*dr, (i+1)*dr], r = hypot(x, y) of the exit point */
+    float dr;
+    double* rd_r;           /* device [nr], v_z < 0 exits (reference radialDistance on Rd_index) or NULL */
+    double* tt_r;           /* device [nr], v_z > 0 exits or NULL                               */
+    double* rd_top_r;       /* device [nr], top-face exits (virtualOBD.py:165 detector) or NULL */
+    int32_t na;             /* angular bins of width pi/(2 na) over acos|v_z|                  */
+    double* rd_a;           /* device [na] or NULL (reference angularyResolved on Rd)          */
+    double* tt_a;           /* device [na] or NULL                                             */
+    int32_t nl;             /* side line bins (i*dl, (i+1)*dl] over exit z                     */
+    float dl;
+    float line_y;           /* |y| < line_y selection (reference lineDistance y_range)         */
+    double* line_z;         /* device [nl], v_x < 0 exits (virtualOBD.py:171-172) or NULL      */
+    int32_t n_rings;        /* annular top-face detectors                                      */
+    const float* rings;     /* device [2*n_rings]: (r_in, r_out], mm                           */
+    double* ring_w;         /* device [n_rings] or NULL                                        */
+    float* exit_state;      /* device [7][n_packets] (x, y, z, vx, vy, vz, w) in the reference's
+                               get_result encoding, indexed by packet id - first_packet, or NULL */
+} pmo_tallies;
+
+/* ---- library ------------------------------------------------------------------------ */
+int pmo_abi_version(void);
+const char* pmo_last_error(void);
+size_t pmo_workspace_bytes(void);
+/* number of compute units and the persistent grid the walk would use for a block size */
+int pmo_device_info(int32_t* n_cu, int32_t* blocks_per_cu_walk, int32_t block_size);
+
+/* One walk of run->n_packets packets through `vol`.
+ * Replaces cuVMC (pymopt/voxel_gpu/_cukernel.py:27-227) plus the host tallies built on its
+ * output: getRdTtRate (pymopt/voxel_gpu/_classes.py:326-334), radialDistance /
+ * angularyResolved / lineDistance (pymopt/metrics/_scoring.py:16-51) and the virtualOBD
+ * detectors (pymopt/virtualOBD.py:158-172). */
+int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run,
+             const pmo_tallies* tal, void* workspace, void* stream);
+
+/* Diagnostic: out[4*i .. 4*i+3] = Philox4x32-10(ctr[4*i .. 4*i+3], key) on the device, the
+ * generator every walk draw comes from (no reference counterpart: the reference draws from
+ * curand XORWOW, pymopt/voxel_gpu/_cukernel.py:42-43). ctr and out are device arrays of
+ * 4*n uint32. */
+int pmo_philox(const uint32_t* ctr, uint32_t key0, uint32_t key1, uint32_t* out, int64_t n,
+               void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PMO_H */

@@ -1,0 +1,247 @@
+"""TEST INFRASTRUCTURE ONLY -- ctypes front end of the CPU oracle (oracle.c).
+
+The oracle restates pyMonteOpt's voxel walk (pymopt/voxel/_kernel.py:23-172) in C with
+FP64 tallies. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this module, and only as the checker; the product package never does.
+
+`walk()` returns the same raw-tally dictionary layout as the HIP path
+(pymonteopt_amd.runtime.run_walk), so the two can be compared key by key.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "_build", "liboracle.so")
+
+SCALAR_NAMES = ["Rd", "Tt", "other", "A", "kill", "gain", "w0",
+                "face_x-", "face_x+", "face_y-", "face_y+", "face_z-", "face_z+"]
+EVENT_NAMES = ["packets", "interactions", "crossings", "fresnel", "roulette", "killed", "escaped"]
+
+
+class OgVolume(C.Structure):
+    _fields_ = [("labels", C.c_void_p), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("n_labels", C.c_int32), ("pitch", C.c_double),
+                ("mua", C.c_void_p), ("mus", C.c_void_p), ("g", C.c_void_p), ("n", C.c_void_p)]
+
+
+class OgSource(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("ix", C.c_int32), ("iy", C.c_int32), ("iz", C.c_int32),
+                ("radius", C.c_double), ("dir", C.c_double * 3), ("w0", C.c_double)]
+
+
+class OgRun(C.Structure):
+    _fields_ = [("n_packets", C.c_uint64), ("first_packet", C.c_uint64),
+                ("seed", C.c_uint32), ("stream", C.c_uint32),
+                ("wth", C.c_double), ("roulette_m", C.c_double),
+                ("flags", C.c_uint32), ("n_threads", C.c_int32)]
+
+
+class OgTallies(C.Structure):
+    _fields_ = [("scalars", C.c_double * 13), ("events", C.c_uint64 * 7),
+                ("absorbed", C.c_void_p), ("nr", C.c_int32), ("dr", C.c_double),
+                ("rd_r", C.c_void_p), ("tt_r", C.c_void_p), ("rd_top_r", C.c_void_p),
+                ("na", C.c_int32), ("rd_a", C.c_void_p), ("tt_a", C.c_void_p),
+                ("nl", C.c_int32), ("dl", C.c_double), ("line_y", C.c_double),
+                ("line_z", C.c_void_p), ("n_rings", C.c_int32), ("rings", C.c_void_p),
+                ("ring_w", C.c_void_p), ("exit_state", C.c_void_p),
+                ("rd_wl", C.c_void_p), ("rd_wn", C.c_void_p)]
+
+
+class OgEvent(C.Structure):
+    _fields_ = [("type", C.c_int32), ("ax", C.c_int32), ("ay", C.c_int32), ("az", C.c_int32),
+                ("p", C.c_double * 3), ("v", C.c_double * 3), ("w", C.c_double)]
+
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    if force or not os.path.exists(_SO):
+        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _SO
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        L = C.CDLL(_SO)
+        L.og_walk_f64.argtypes = [C.POINTER(OgVolume), C.POINTER(OgSource), C.POINTER(OgRun),
+                                  C.POINTER(OgTallies)]
+        L.og_walk_f32.argtypes = L.og_walk_f64.argtypes
+        L.og_walk_f64.restype = C.c_int
+        L.og_walk_f32.restype = C.c_int
+        L.og_trace_f64.argtypes = [C.POINTER(OgVolume), C.POINTER(OgSource), C.POINTER(OgRun),
+                                   C.c_uint64, C.POINTER(OgEvent), C.c_int32]
+        L.og_trace_f64.restype = C.c_int
+        L.og_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.og_philox4x32_10.restype = None
+        L.og_u01.argtypes = [C.c_uint32]
+        L.og_u01.restype = C.c_double
+        L.og_fresnel.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
+        L.og_fresnel.restype = C.c_double
+        L.og_hg_cos.argtypes = [C.c_double, C.c_double]
+        L.og_hg_cos.restype = C.c_double
+        L.og_spin.argtypes = [C.POINTER(C.c_double), C.c_double, C.c_double]
+        L.og_spin.restype = None
+        L.og_boundary.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double,
+                                  C.POINTER(C.c_int32)]
+        L.og_boundary.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+# ---------------------------------------------------------------- primitives
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*[int(x) & 0xFFFFFFFF for x in ctr])
+    k = (C.c_uint32 * 2)(*[int(x) & 0xFFFFFFFF for x in key])
+    o = (C.c_uint32 * 4)()
+    lib().og_philox4x32_10(c, k, o)
+    return tuple(int(x) for x in o)
+
+
+def u01(x):
+    return lib().og_u01(int(x) & 0xFFFFFFFF)
+
+
+def fresnel(ni, nt, abs_vk):
+    ct = C.c_double()
+    r = lib().og_fresnel(float(ni), float(nt), float(abs_vk), C.byref(ct))
+    return r, ct.value
+
+
+def hg_cos(g, u):
+    return lib().og_hg_cos(float(g), float(u))
+
+
+def spin(v, cos_th, phi):
+    a = (C.c_double * 3)(*[float(x) for x in v])
+    lib().og_spin(a, float(cos_th), float(phi))
+    return np.array(list(a))
+
+
+def boundary(p, v, half):
+    pa = (C.c_double * 3)(*[float(x) for x in p])
+    va = (C.c_double * 3)(*[float(x) for x in v])
+    k = C.c_int32()
+    d = lib().og_boundary(pa, va, float(half), C.byref(k))
+    return d, k.value
+
+
+# ---------------------------------------------------------------- whole walk
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _volume(labels, pitch, mua, mus, g, n):
+    labels = np.ascontiguousarray(labels, dtype=np.uint8)
+    mua = np.ascontiguousarray(mua, dtype=np.float64)
+    mus = np.ascontiguousarray(mus, dtype=np.float64)
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    nlab = int(mua.size)
+    assert n.size == nlab + 1 and mus.size == nlab and g.size == nlab
+    v = OgVolume(labels.ctypes.data, labels.shape[0], labels.shape[1], labels.shape[2], nlab,
+                 float(pitch), mua.ctypes.data, mus.ctypes.data, g.ctypes.data, n.ctypes.data)
+    return v, (labels, mua, mus, g, n)
+
+
+def _source(src):
+    s = OgSource()
+    s.kind = int(src.get("kind", 0))
+    s.ix, s.iy, s.iz = (int(x) for x in src["voxel"])
+    s.radius = float(src.get("radius", 0.0))
+    d = src.get("dir", (0.0, 0.0, 1.0))
+    for i in range(3):
+        s.dir[i] = float(d[i])
+    s.w0 = float(src["w0"])
+    return s
+
+
+def walk(labels, pitch, mua, mus, g, n, source, n_packets, seed, *, stream=0, first_packet=0,
+         wth=1e-4, roulette_m=10.0, flags=0, nr=0, dr=0.1, na=0, nl=0, dl=0.1, line_y=5.0,
+         rings=None, absorbed=True, exit_state=False, path_stats=False, precision="f64",
+         threads=0):
+    """Run the oracle walk; returns the raw tally dictionary (unnormalised sums)."""
+    vol, keep = _volume(labels, pitch, mua, mus, g, n)
+    src = _source(source)
+    run = OgRun(int(n_packets), int(first_packet), int(seed) & 0xFFFFFFFF, int(stream) & 0xFFFFFFFF,
+                float(wth), float(roulette_m), int(flags), int(threads))
+    t = OgTallies()
+    nvox = vol.nx * vol.ny * vol.nz
+    out = {}
+    bufs = {}
+    if absorbed:
+        bufs["absorbed"] = np.zeros(nvox, np.float64)
+    if nr > 0:
+        for k in ("rd_r", "tt_r", "rd_top_r"):
+            bufs[k] = np.zeros(nr, np.float64)
+        if path_stats:
+            bufs["rd_wl"] = np.zeros(nr * vol.n_labels, np.float64)
+            bufs["rd_wn"] = np.zeros(nr * vol.n_labels, np.float64)
+    if na > 0:
+        bufs["rd_a"] = np.zeros(na, np.float64)
+        bufs["tt_a"] = np.zeros(na, np.float64)
+    if nl > 0:
+        bufs["line_z"] = np.zeros(nl, np.float64)
+    ring_arr = None
+    if rings is not None and len(rings) > 0:
+        ring_arr = np.ascontiguousarray(np.asarray(rings, np.float64).reshape(-1))
+        bufs["ring_w"] = np.zeros(ring_arr.size // 2, np.float64)
+    if exit_state:
+        bufs["exit_state"] = np.zeros(7 * int(n_packets), np.float32)
+    t.absorbed = _ptr(bufs.get("absorbed"))
+    t.nr, t.dr = int(nr), float(dr)
+    t.rd_r, t.tt_r, t.rd_top_r = _ptr(bufs.get("rd_r")), _ptr(bufs.get("tt_r")), _ptr(bufs.get("rd_top_r"))
+    t.na = int(na)
+    t.rd_a, t.tt_a = _ptr(bufs.get("rd_a")), _ptr(bufs.get("tt_a"))
+    t.nl, t.dl, t.line_y = int(nl), float(dl), float(line_y)
+    t.line_z = _ptr(bufs.get("line_z"))
+    t.n_rings = 0 if ring_arr is None else ring_arr.size // 2
+    t.rings = _ptr(ring_arr)
+    t.ring_w = _ptr(bufs.get("ring_w"))
+    t.exit_state = _ptr(bufs.get("exit_state"))
+    t.rd_wl, t.rd_wn = _ptr(bufs.get("rd_wl")), _ptr(bufs.get("rd_wn"))
+    fn = lib().og_walk_f64 if precision == "f64" else lib().og_walk_f32
+    rc = fn(C.byref(vol), C.byref(src), C.byref(run), C.byref(t))
+    if rc != 0:
+        raise RuntimeError(f"oracle walk failed with code {rc}")
+    out["scalars"] = dict(zip(SCALAR_NAMES, [float(x) for x in t.scalars]))
+    out["events"] = dict(zip(EVENT_NAMES, [int(x) for x in t.events]))
+    for k, v in bufs.items():
+        out[k] = v
+    if "absorbed" in out:
+        out["absorbed"] = out["absorbed"].reshape(vol.nx, vol.ny, vol.nz)
+    if "exit_state" in out:
+        out["exit_state"] = out["exit_state"].reshape(7, int(n_packets))
+    if path_stats and nr > 0:
+        out["rd_wl"] = out["rd_wl"].reshape(nr, vol.n_labels)
+        out["rd_wn"] = out["rd_wn"].reshape(nr, vol.n_labels)
+    out["n_packets"] = int(n_packets)
+    return out
+
+
+def trace(labels, pitch, mua, mus, g, n, source, pid, seed, *, stream=0, wth=1e-4, roulette_m=10.0,
+          flags=0, cap=100000):
+    """Event trace of one packet (FP64 oracle)."""
+    vol, keep = _volume(labels, pitch, mua, mus, g, n)
+    src = _source(source)
+    run = OgRun(1, int(pid), int(seed), int(stream), float(wth), float(roulette_m), int(flags), 1)
+    ev = (OgEvent * cap)()
+    m = lib().og_trace_f64(C.byref(vol), C.byref(src), C.byref(run), int(pid), ev, int(cap))
+    if m < 0:
+        raise RuntimeError("trace failed")
+    rows = []
+    for i in range(m):
+        e = ev[i]
+        rows.append(dict(type=e.type, a=(e.ax, e.ay, e.az), p=tuple(e.p), v=tuple(e.v), w=e.w))
+    return rows

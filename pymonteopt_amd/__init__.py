@@ -1,0 +1,22 @@
+"""pymonteopt_amd -- pyMonteOpt's voxel Monte Carlo photon walk, MI355X-native.
+
+The walk runs in hand-written HIP for gfx950 (pymonteopt_amd/libpmo.so, C ABI in
+include/pmo.h); this package is the host side: the pymopt voxel-model API (voxel.py), the
+geometry store (geometry.py), the runtime (runtime.py), the batched sweep (batched.py) and
+the differentiable walk (adjoint.py).
+"""
+from .geometry import Source, bone_optics, bone_phantom, optics_table, plate_volume  # noqa: F401
+
+__version__ = "0.1.0"
+
+
+def __getattr__(name):
+    # the GPU-facing modules import torch; load them on first use
+    if name in ("VoxelPlateModel", "VoxelLabelModel", "PlateModel", "LabelVolumeModel", "set_params",
+                "BaseVoxelMonteCarlo"):
+        from . import voxel
+        return getattr(voxel, name)
+    if name in ("Walker", "TallySpec", "TallySet"):
+        from . import runtime
+        return getattr(runtime, name)
+    raise AttributeError(name)

@@ -1,0 +1,105 @@
+"""ctypes mirror of include/pmo.h and the loader of pymonteopt_amd/libpmo.so.
+
+The library is loaded from the package directory only. If it is missing the import of
+the walk fails loudly: there is no CPU fallback on the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpmo.so")
+
+PMO_OK = 0
+PMO_ERR_INVALID = -1
+PMO_ERR_LABEL = -2
+PMO_ERR_HIP = -3
+PMO_ERR_UNSUPPORTED = -4
+
+PMO_BEAM_PENCIL = 0
+PMO_BEAM_FLAT = 1
+PMO_BEAM_GAUSS = 2
+
+PMO_F_FP64 = 1 << 0
+PMO_F_NORMAL_FRESNEL = 1 << 1
+
+SCALAR_NAMES = ["Rd", "Tt", "other", "A", "kill", "gain", "w0",
+                "face_x-", "face_x+", "face_y-", "face_y+", "face_z-", "face_z+"]
+PMO_S_COUNT = 13
+EVENT_NAMES = ["packets", "interactions", "crossings", "fresnel", "roulette", "killed", "escaped"]
+PMO_EV_COUNT = 7
+
+
+class PmoVolume(C.Structure):
+    _fields_ = [("labels", C.c_void_p), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("n_labels", C.c_int32), ("pitch", C.c_float), ("optics", C.c_void_p)]
+
+
+class PmoSource(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("ix", C.c_int32), ("iy", C.c_int32), ("iz", C.c_int32),
+                ("radius", C.c_float), ("dir", C.c_float * 3), ("w0", C.c_float)]
+
+
+class PmoRun(C.Structure):
+    _fields_ = [("n_packets", C.c_uint64), ("first_packet", C.c_uint64),
+                ("seed", C.c_uint32), ("stream", C.c_uint32),
+                ("wth", C.c_float), ("roulette_m", C.c_float), ("flags", C.c_uint32),
+                ("block_size", C.c_int32), ("grid_blocks", C.c_int32), ("packets_per_lane", C.c_int32)]
+
+
+class PmoTallies(C.Structure):
+    _fields_ = [("scalars", C.c_void_p), ("events", C.c_void_p), ("absorbed", C.c_void_p),
+                ("nr", C.c_int32), ("dr", C.c_float),
+                ("rd_r", C.c_void_p), ("tt_r", C.c_void_p), ("rd_top_r", C.c_void_p),
+                ("na", C.c_int32), ("rd_a", C.c_void_p), ("tt_a", C.c_void_p),
+                ("nl", C.c_int32), ("dl", C.c_float), ("line_y", C.c_float), ("line_z", C.c_void_p),
+                ("n_rings", C.c_int32), ("rings", C.c_void_p), ("ring_w", C.c_void_p),
+                ("exit_state", C.c_void_p)]
+
+
+# every symbol include/pmo.h declares, with its ctypes signature
+SIGNATURES = {
+    "pmo_abi_version": ([], C.c_int),
+    "pmo_last_error": ([], C.c_char_p),
+    "pmo_workspace_bytes": ([], C.c_size_t),
+    "pmo_device_info": ([C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32], C.c_int),
+    "pmo_walk": ([C.POINTER(PmoVolume), C.POINTER(PmoSource), C.POINTER(PmoRun), C.POINTER(PmoTallies),
+                  C.c_void_p, C.c_void_p], C.c_int),
+    "pmo_philox": ([C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
+}
+
+_lib = None
+
+
+class PmoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libpmo error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libpmo.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m pymonteopt_amd.build` "
+                "(the walk has no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        if L.pmo_abi_version() != 1:
+            raise ImportError("libpmo ABI version mismatch")
+        _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc != PMO_OK:
+        msg = lib().pmo_last_error().decode(errors="replace")
+        if rc == PMO_ERR_INVALID or rc == PMO_ERR_LABEL:
+            raise ValueError(f"libpmo error {rc}: {msg}")
+        raise PmoError(rc, msg)

@@ -1,0 +1,47 @@
+// Counter-based Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) for the photon walk.
+//
+// Draw assignment (shared with the CPU oracle, so both see the same numbers):
+//   counter = (block, tag, packet_id_lo, packet_id_hi), key = (seed, stream)
+//   tag 0, block i : the four uniforms of hop/interaction i of the packet
+//                    (.x step length, .y roulette, .z HG cos(theta), .w azimuth)
+//   tag 1, block j : Fresnel draws 4j..4j+3 (one per refractive-index change)
+//   tag 2, block 0 : launch position of the flat-disc / Gaussian beams
+// A packet's whole history is a pure function of (seed, stream, packet id), which is what
+// the adjoint's path replay and the batched/multi-call sharding rely on.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pmo {
+
+constexpr uint32_t kPhiloxM0 = 0xD2511F53u;
+constexpr uint32_t kPhiloxM1 = 0xCD9E8D57u;
+constexpr uint32_t kPhiloxW0 = 0x9E3779B9u;
+constexpr uint32_t kPhiloxW1 = 0xBB67AE85u;
+
+enum : uint32_t { kTagWalk = 0u, kTagFresnel = 1u, kTagSource = 2u };
+
+__host__ __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t m0 = (uint64_t)kPhiloxM0 * c.x;
+        const uint64_t m1 = (uint64_t)kPhiloxM1 * c.z;
+        c = make_uint4((uint32_t)(m1 >> 32) ^ c.y ^ k0, (uint32_t)m1,
+                       (uint32_t)(m0 >> 32) ^ c.w ^ k1, (uint32_t)m0);
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
+    }
+    return c;
+}
+
+// uniform on (0, 1]: top 24 bits plus one, times 2^-24; exact in float and double, never 0
+// so -log(u) is finite (the reference's curand_uniform range, _cukernel.py:75).
+template <typename R>
+__host__ __device__ __forceinline__ R u01(uint32_t x)
+{
+    return (R)((x >> 8) + 1u) * (R)(1.0 / 16777216.0);
+}
+
+}  // namespace pmo

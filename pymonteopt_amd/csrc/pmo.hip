@@ -1,0 +1,238 @@
+// libpmo.so: C ABI of the MI355X voxel photon walk (declarations and reference
+// citations in include/pmo.h). Host code here validates arguments, sizes the persistent
+// grid and the dynamic LDS layout, and launches on the caller's stream. It allocates
+// nothing: the only per-call scratch is the caller's workspace.
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+
+#include "pmo.h"
+#include "walk.hpp"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what)
+{
+    return fail(PMO_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+constexpr size_t kWorkspaceBytes = 256;
+constexpr int kDefaultBlock = 256;
+constexpr int kDefaultPacketsPerLane = 4;
+constexpr uint32_t kMaxEvents = 1u << 28;
+constexpr size_t kMaxLds = 64 * 1024;
+
+struct DevInfo {
+    int n_cu = 0;
+    bool ok = false;
+};
+
+std::mutex g_mu;
+DevInfo g_dev[64];
+
+int device_info(DevInfo* out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= 64) return fail(PMO_ERR_UNSUPPORTED, "device ordinal %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_dev[dev].ok) {
+        int n = 0;
+        e = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return hip_fail(e, "hipDeviceGetAttribute");
+        g_dev[dev].n_cu = n;
+        g_dev[dev].ok = true;
+    }
+    *out = g_dev[dev];
+    return PMO_OK;
+}
+
+template <typename R>
+int blocks_per_cu(int block, size_t lds, int* out)
+{
+    int nb = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::walk_kernel<R>, block, lds);
+    if (e != hipSuccess) return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    *out = nb > 0 ? nb : 1;
+    return PMO_OK;
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
+
+extern "C" {
+
+int pmo_abi_version(void) { return PMO_ABI_VERSION; }
+
+const char* pmo_last_error(void) { return g_err; }
+
+size_t pmo_workspace_bytes(void) { return kWorkspaceBytes; }
+
+int pmo_device_info(int32_t* n_cu, int32_t* blocks_per_cu_walk, int32_t block_size)
+{
+    DevInfo d;
+    int rc = device_info(&d);
+    if (rc != PMO_OK) return rc;
+    if (n_cu) *n_cu = d.n_cu;
+    if (blocks_per_cu_walk) {
+        const int bs = block_size > 0 ? block_size : kDefaultBlock;
+        int nb = 0;
+        rc = blocks_per_cu<float>(bs, 4096, &nb);
+        if (rc != PMO_OK) return rc;
+        *blocks_per_cu_walk = nb;
+    }
+    return PMO_OK;
+}
+
+int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, const pmo_tallies* tal,
+             void* workspace, void* stream)
+{
+    g_err[0] = '\0';
+    if (!vol || !src || !run || !tal) return fail(PMO_ERR_INVALID, "null argument");
+    if (!vol->labels || !vol->optics) return fail(PMO_ERR_INVALID, "volume labels/optics pointer is null");
+    if (!tal->scalars) return fail(PMO_ERR_INVALID, "tallies.scalars is required");
+    if (!workspace) return fail(PMO_ERR_INVALID, "workspace is null");
+    if (vol->nx < 3 || vol->ny < 3 || vol->nz < 3)
+        return fail(PMO_ERR_INVALID, "volume extents (%d, %d, %d) must be >= 3", vol->nx, vol->ny, vol->nz);
+    const unsigned long long nvox = (unsigned long long)vol->nx * vol->ny * vol->nz;
+    if (nvox >= (1ull << 31)) return fail(PMO_ERR_UNSUPPORTED, "volume of %llu voxels exceeds 2^31", nvox);
+    if (vol->n_labels < 1 || vol->n_labels > 255)
+        return fail(PMO_ERR_LABEL, "n_labels = %d outside [1, 255]", vol->n_labels);
+    if (!(vol->pitch > 0.f)) return fail(PMO_ERR_INVALID, "pitch must be > 0");
+    if (src->kind < PMO_BEAM_PENCIL || src->kind > PMO_BEAM_GAUSS)
+        return fail(PMO_ERR_INVALID, "unknown beam kind %d", src->kind);
+    if (src->ix < 1 || src->ix > vol->nx - 2 || src->iy < 1 || src->iy > vol->ny - 2 || src->iz < 1 ||
+        src->iz > vol->nz - 2)
+        return fail(PMO_ERR_INVALID, "launch voxel (%d, %d, %d) is not inside the volume interior", src->ix,
+                    src->iy, src->iz);
+    const float dn = src->dir[0] * src->dir[0] + src->dir[1] * src->dir[1] + src->dir[2] * src->dir[2];
+    if (!(dn > 0.999f && dn < 1.001f)) return fail(PMO_ERR_INVALID, "launch direction is not a unit vector");
+    if (!(run->roulette_m >= 1.f)) return fail(PMO_ERR_INVALID, "roulette_m must be >= 1");
+    if (run->n_packets == 0) return PMO_OK;
+    const int bs = run->block_size > 0 ? run->block_size : kDefaultBlock;
+    if (bs % 64 != 0 || bs > 1024) return fail(PMO_ERR_INVALID, "block_size %d must be a multiple of 64, <= 1024", bs);
+    const int ppl = run->packets_per_lane > 0 ? run->packets_per_lane : kDefaultPacketsPerLane;
+    if (ppl > 4096) return fail(PMO_ERR_INVALID, "packets_per_lane %d too large", ppl);
+    if (tal->nr < 0 || tal->na < 0 || tal->nl < 0 || tal->n_rings < 0)
+        return fail(PMO_ERR_INVALID, "negative histogram size");
+    if (tal->nr > 0 && !(tal->dr > 0.f)) return fail(PMO_ERR_INVALID, "dr must be > 0");
+    if (tal->nl > 0 && !(tal->dl > 0.f)) return fail(PMO_ERR_INVALID, "dl must be > 0");
+    if (tal->n_rings > 0 && !tal->rings) return fail(PMO_ERR_INVALID, "rings pointer is null");
+    const bool fp64 = (run->flags & PMO_F_FP64) != 0u;
+
+    DevInfo d;
+    int rc = device_info(&d);
+    if (rc != PMO_OK) return rc;
+
+    pmo::WalkParams P;
+    memset(&P, 0, sizeof(P));
+    P.labels = vol->labels;
+    P.optics = reinterpret_cast<const float4*>(vol->optics);
+    P.nx = (uint32_t)vol->nx; P.ny = (uint32_t)vol->ny; P.nz = (uint32_t)vol->nz;
+    P.nvox = (uint32_t)nvox;
+    P.sx = P.ny * P.nz; P.sy = P.nz;
+    P.n_labels = (uint32_t)vol->n_labels;
+    P.cx = (vol->nx - 1) / 2; P.cy = (vol->ny - 1) / 2;
+    P.pitch = vol->pitch;
+    P.src_kind = src->kind; P.ix = src->ix; P.iy = src->iy; P.iz = src->iz;
+    P.radius = src->radius; P.dirx = src->dir[0]; P.diry = src->dir[1]; P.dirz = src->dir[2];
+    P.w0 = src->w0;
+    P.n_packets = run->n_packets; P.first_packet = run->first_packet;
+    P.seed = run->seed; P.stream = run->stream;
+    P.wth = run->wth; P.roulette_m = run->roulette_m; P.inv_m = 1.f / run->roulette_m;
+    P.flags = run->flags;
+    P.batch = (uint32_t)(64 * ppl);
+    P.max_events = kMaxEvents;
+    P.scalars = tal->scalars;
+    P.events = reinterpret_cast<unsigned long long*>(tal->events);
+    P.absorbed = tal->absorbed;
+    P.nr = tal->nr; P.dr = tal->dr;
+    P.rd_r = tal->nr > 0 ? tal->rd_r : nullptr;
+    P.tt_r = tal->nr > 0 ? tal->tt_r : nullptr;
+    P.rd_top_r = tal->nr > 0 ? tal->rd_top_r : nullptr;
+    P.na = tal->na; P.da = tal->na > 0 ? 1.57079632679489662f / (float)tal->na : 1.f;
+    P.rd_a = tal->na > 0 ? tal->rd_a : nullptr;
+    P.tt_a = tal->na > 0 ? tal->tt_a : nullptr;
+    P.nl = tal->nl; P.dl = tal->dl; P.line_y = tal->line_y;
+    P.line_z = tal->nl > 0 ? tal->line_z : nullptr;
+    P.n_rings = tal->n_rings; P.rings = tal->rings;
+    P.ring_w = tal->n_rings > 0 ? tal->ring_w : nullptr;
+    P.exit_state = tal->exit_state;
+    P.work = reinterpret_cast<unsigned long long*>(workspace);
+
+    // dynamic LDS: label table | block scalars | block events | histograms
+    const size_t tab_bytes = (size_t)(vol->n_labels + 1) * (fp64 ? sizeof(pmo::LabelConst<double>)
+                                                                  : sizeof(pmo::LabelConst<float>));
+    P.lds_tab = 0;
+    P.lds_scal = (uint32_t)align_up(tab_bytes, 16);
+    P.lds_ev = P.lds_scal + (uint32_t)align_up(pmo::kBS_COUNT * sizeof(double), 16);
+    P.lds_hist = P.lds_ev + (uint32_t)align_up(pmo::kBE_COUNT * sizeof(unsigned long long), 16);
+    uint32_t off = 0;
+    P.h_rd = off; off += P.rd_r ? (uint32_t)P.nr : 0u;
+    P.h_tt = off; off += P.tt_r ? (uint32_t)P.nr : 0u;
+    P.h_top = off; off += P.rd_top_r ? (uint32_t)P.nr : 0u;
+    P.h_rda = off; off += P.rd_a ? (uint32_t)P.na : 0u;
+    P.h_tta = off; off += P.tt_a ? (uint32_t)P.na : 0u;
+    P.h_line = off; off += P.line_z ? (uint32_t)P.nl : 0u;
+    P.h_ring = off; off += P.ring_w ? (uint32_t)P.n_rings : 0u;
+    P.h_total = off;
+    const size_t lds = P.lds_hist + (size_t)P.h_total * sizeof(double);
+    if (lds > kMaxLds)
+        return fail(PMO_ERR_UNSUPPORTED, "exit histograms need %zu B of LDS (limit %zu); use fewer bins", lds, kMaxLds);
+
+    int nb = 0;
+    rc = fp64 ? blocks_per_cu<double>(bs, lds, &nb) : blocks_per_cu<float>(bs, lds, &nb);
+    if (rc != PMO_OK) return rc;
+    unsigned long long grid = run->grid_blocks > 0 ? (unsigned long long)run->grid_blocks
+                                                   : (unsigned long long)nb * (unsigned long long)d.n_cu;
+    const unsigned long long need_blocks = (run->n_packets + P.batch - 1) / P.batch;
+    const unsigned long long waves_per_block = (unsigned long long)bs / 64;
+    const unsigned long long useful = (need_blocks + waves_per_block - 1) / waves_per_block;
+    if (grid > useful) grid = useful;
+    if (grid < 1) grid = 1;
+
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(pmo::zero_u64_kernel, dim3(1), dim3(64), 0, s, P.work, 4);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "zero workspace launch");
+    if (fp64)
+        hipLaunchKernelGGL(pmo::walk_kernel<double>, dim3((unsigned)grid), dim3(bs), lds, s, P);
+    else
+        hipLaunchKernelGGL(pmo::walk_kernel<float>, dim3((unsigned)grid), dim3(bs), lds, s, P);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "walk launch");
+    return PMO_OK;
+}
+
+int pmo_philox(const uint32_t* ctr, uint32_t key0, uint32_t key1, uint32_t* out, int64_t n, void* stream)
+{
+    g_err[0] = '\0';
+    if (n < 0) return fail(PMO_ERR_INVALID, "n < 0");
+    if (n == 0) return PMO_OK;
+    if (!ctr || !out) return fail(PMO_ERR_INVALID, "null pointer");
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(pmo::philox_kernel, dim3(blocks), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       reinterpret_cast<const uint4*>(ctr), key0, key1, reinterpret_cast<uint4*>(out), n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "philox launch");
+    return PMO_OK;
+}
+
+}  // extern "C"

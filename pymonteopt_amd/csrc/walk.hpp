@@ -1,0 +1,525 @@
+// Device side of the voxel photon walk for gfx950 (CDNA4, wave64).
+//
+// One packet per lane, persistent grid. Each loop iteration a lane performs one event of
+// pymopt/voxel/_kernel.py:60-165: either a voxel-face crossing (with Fresnel test where
+// the refractive index changes) or the end of a hop inside a voxel (drop, roulette,
+// Henyey-Greenstein spin, next step length). A lane whose packet ended takes the next
+// packet id from its wave's batch; a wave claims batches of ids from one global counter.
+//
+// Optics (per label) live in LDS; labels are read as bytes through L2; absorbed weight
+// is coalesced per lane while consecutive deposits hit the same voxel and flushed with
+// FP64 atomics; exit tallies go to per-block FP64 LDS histograms flushed once at the end.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "philox.hpp"
+
+namespace pmo {
+
+// ------------------------------------------------------------------ precision traits
+template <typename R> struct Mth;
+
+template <> struct Mth<float> {
+    static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    static __device__ __forceinline__ float rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+    static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ float neglog(float u) { return -__logf(u); }
+    static __device__ __forceinline__ void sincos2pi(float u, float* s, float* c)
+    {
+        const float a = 6.28318530717958647f * u;
+        *s = __sinf(a);
+        *c = __cosf(a);
+    }
+    static __device__ __forceinline__ float fabs(float x) { return __builtin_fabsf(x); }
+    static __device__ __forceinline__ float copysign(float a, float b) { return __builtin_copysignf(a, b); }
+    static __device__ __forceinline__ float fmax(float a, float b) { return __builtin_fmaxf(a, b); }
+};
+
+template <> struct Mth<double> {
+    static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+    static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+    static __device__ __forceinline__ double rsqrt(double x) { return 1.0 / ::sqrt(x); }
+    static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+    static __device__ __forceinline__ double neglog(double u) { return -::log(u); }
+    static __device__ __forceinline__ void sincos2pi(double u, double* s, double* c)
+    {
+        ::sincos(6.283185307179586477 * u, s, c);
+    }
+    static __device__ __forceinline__ double fabs(double x) { return __builtin_fabs(x); }
+    static __device__ __forceinline__ double copysign(double a, double b) { return __builtin_copysign(a, b); }
+    static __device__ __forceinline__ double fmax(double a, double b) { return __builtin_fmax(a, b); }
+};
+
+// per-label constants in LDS, derived once per block from (mu_a, mu_s, g, n)
+template <typename R> struct LabelConst {
+    R mt;      // mu_a + mu_s   (reference mt, _kernel.py:56)
+    R ra;      // mu_a / mt     (drop fraction, _kernel.py:130)
+    R n;       // refractive index
+    R g;
+    R hg1;     // 1 + g^2
+    R hg2;     // 1 - g^2
+    R hg3;     // 1 - g
+    R hg4;     // 2 g
+    R hginv;   // 1 / (2 g)
+    R pad[3];
+};
+
+// ------------------------------------------------------------------ kernel parameters
+struct WalkParams {
+    const uint8_t* labels;
+    const float4* optics;
+    uint32_t nx, ny, nz, nvox;
+    uint32_t sx, sy;           // strides of x and y in the C-order volume
+    uint32_t n_labels;         // end label
+    int32_t cx, cy;            // encoder centre voxel
+    float pitch;
+    // source
+    int32_t src_kind, ix, iy, iz;
+    float radius, dirx, diry, dirz, w0;
+    // run
+    unsigned long long n_packets, first_packet;
+    uint32_t seed, stream;
+    float wth, roulette_m, inv_m;
+    uint32_t flags;
+    uint32_t batch;            // packet ids claimed per wave per counter atomic
+    uint32_t max_events;       // per-packet event guard
+    // tallies
+    double* scalars;
+    unsigned long long* events;
+    double* absorbed;
+    int32_t nr;
+    float dr;
+    double* rd_r;
+    double* tt_r;
+    double* rd_top_r;
+    int32_t na;
+    float da;
+    double* rd_a;
+    double* tt_a;
+    int32_t nl;
+    float dl, line_y;
+    double* line_z;
+    int32_t n_rings;
+    const float* rings;
+    double* ring_w;
+    float* exit_state;
+    unsigned long long* work;
+    // dynamic LDS layout (bytes)
+    uint32_t lds_tab, lds_scal, lds_ev, lds_hist;
+    uint32_t h_rd, h_tt, h_top, h_rda, h_tta, h_line, h_ring, h_total;  // offsets in doubles
+};
+
+// block-level accumulator slots in LDS (doubles)
+enum { kBS_RD = 0, kBS_TT, kBS_OTHER, kBS_ABS, kBS_KILL, kBS_GAIN, kBS_W0, kBS_FACE, kBS_COUNT = 13 };
+enum { kBE_PACKETS = 0, kBE_INTERACT, kBE_CROSS, kBE_FRESNEL, kBE_ROULETTE, kBE_KILLED, kBE_ESCAPED, kBE_COUNT = 7 };
+
+__device__ __forceinline__ void lds_add(double* p, double v) { atomicAdd(p, v); }
+__device__ __forceinline__ void lds_add(unsigned long long* p, unsigned long long v) { atomicAdd(p, v); }
+
+__device__ __forceinline__ void gmem_add(double* p, double v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int bin_of(float x, float width, int nbins)
+{
+    // bins (i*width, (i+1)*width]; x <= 0 in none (metrics/_scoring.py:22 uses alpha[i] < r)
+    if (!(x > 0.f)) return -1;
+    const int b = (int)__builtin_ceilf(x / width) - 1;
+    return (b >= 0 && b < nbins) ? b : -1;
+}
+
+// ------------------------------------------------------------------ the walk kernel
+template <typename R>
+__global__ void __launch_bounds__(1024) walk_kernel(const WalkParams P)
+{
+    using M = Mth<R>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    LabelConst<R>* tab = reinterpret_cast<LabelConst<R>*>(smem + P.lds_tab);
+    double* bscal = reinterpret_cast<double*>(smem + P.lds_scal);
+    unsigned long long* bev = reinterpret_cast<unsigned long long*>(smem + P.lds_ev);
+    double* hist = reinterpret_cast<double*>(smem + P.lds_hist);
+
+    const uint32_t end = P.n_labels;
+    for (uint32_t i = threadIdx.x; i <= end; i += blockDim.x) {
+        const float4 o = P.optics[i];
+        LabelConst<R> c;
+        const R ma = (R)o.x, ms = (R)o.y, g = (R)o.z;
+        c.mt = ma + ms;
+        c.ra = (c.mt > (R)0) ? ma / c.mt : (R)0;
+        c.n = (R)o.w;
+        c.g = g;
+        c.hg1 = (R)1 + g * g;
+        c.hg2 = (R)1 - g * g;
+        c.hg3 = (R)1 - g;
+        c.hg4 = (R)2 * g;
+        c.hginv = (g != (R)0) ? (R)1 / ((R)2 * g) : (R)0;
+        c.pad[0] = c.pad[1] = c.pad[2] = (R)0;
+        tab[i] = c;
+    }
+    for (uint32_t i = threadIdx.x; i < kBS_COUNT; i += blockDim.x) bscal[i] = 0.0;
+    for (uint32_t i = threadIdx.x; i < kBE_COUNT; i += blockDim.x) bev[i] = 0ull;
+    for (uint32_t i = threadIdx.x; i < P.h_total; i += blockDim.x) hist[i] = 0.0;
+    __syncthreads();
+
+    const uint32_t lane = threadIdx.x & 63u;
+    const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+    const R half = (R)P.pitch * (R)0.5;
+    const R pitch = (R)P.pitch;
+    const bool normal_limit = (P.flags & 2u) != 0u;
+
+    // lane state
+    bool alive = false;
+    R px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0, ivx = 0, ivy = 0, ivz = 0;
+    R w = 0, st = 0, mt = 0, ni = 0, u_roul = 0, u_th = 0, u_phi = 0;
+    R fb0 = 0, fb1 = 0, fb2 = 0;
+    uint32_t vi = 0, lab = 0, inter = 0, fres = 0, nev = 0;
+    unsigned long long pid = 0;
+    // lane accumulators (kept across packets)
+    double acc_abs = 0.0;
+    uint32_t dep_vi = 0xffffffffu;
+    double dep_acc = 0.0;
+    uint32_t n_cross = 0, n_pk = 0;
+
+    // wave-uniform work batch
+    unsigned long long b_next = 0, b_end = 0;
+    bool exhausted = false;
+
+    for (;;) {
+        // ---------------------------------------------------------- refill idle lanes
+        const unsigned long long need = __ballot(!alive);
+        if (need != 0ull && !exhausted) {
+            if (b_next == b_end) {
+                unsigned long long claim = 0;
+                if (lane == 0) claim = atomicAdd(P.work, (unsigned long long)P.batch);
+                claim = __shfl(claim, 0);
+                if (claim >= P.n_packets) {
+                    exhausted = true;
+                } else {
+                    b_next = claim;
+                    b_end = claim + P.batch < P.n_packets ? claim + P.batch : P.n_packets;
+                }
+            }
+            if (!exhausted) {
+                const uint32_t avail = (uint32_t)(b_end - b_next);
+                const uint32_t rank = (uint32_t)__popcll(need & lt_mask);
+                if (!alive && rank < avail) {
+                    // ---------------- launch a packet (voxel/_classes.py:107-213)
+                    pid = P.first_packet + b_next + rank;
+                    const uint32_t plo = (uint32_t)pid, phi = (uint32_t)(pid >> 32);
+                    int32_t ax = P.ix, ay = P.iy;
+                    px = 0; py = 0; pz = -half;
+                    if (P.src_kind != 0) {
+                        const uint4 r = philox4x32_10(make_uint4(0u, kTagSource, plo, phi), P.seed, P.stream);
+                        const R u0 = u01<R>(r.x), u1 = u01<R>(r.y);
+                        R rad;
+                        if (P.src_kind == 1) rad = (R)P.radius * M::sqrt(u0);
+                        else rad = (R)(P.radius * 0.5f) * M::sqrt((R)2 * M::neglog(u0));
+                        R sa, ca;
+                        M::sincos2pi(u1, &sa, &ca);
+                        const R x = rad * ca, y = rad * sa;
+                        const int dx = (int)__builtin_floor((double)(x / pitch + (R)0.5));
+                        const int dy = (int)__builtin_floor((double)(y / pitch + (R)0.5));
+                        ax += dx; ay += dy;
+                        px = x - pitch * (R)dx;
+                        py = y - pitch * (R)dy;
+                        // keep the launch voxel inside the padded volume's interior
+                        if (ax < 1) { ax = 1; px = -half; }
+                        if (ax > (int)P.nx - 2) { ax = (int)P.nx - 2; px = half; }
+                        if (ay < 1) { ay = 1; py = -half; }
+                        if (ay > (int)P.ny - 2) { ay = (int)P.ny - 2; py = half; }
+                    }
+                    vi = ((uint32_t)ax * P.ny + (uint32_t)ay) * P.nz + (uint32_t)P.iz;
+                    vx = (R)P.dirx; vy = (R)P.diry; vz = (R)P.dirz;
+                    ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
+                    w = (R)P.w0;
+                    uint32_t l0 = P.labels[vi];
+                    lab = l0 < end ? l0 : end;
+                    ni = tab[lab].n;
+                    mt = tab[lab].mt;
+                    inter = 0; fres = 0; nev = 0;
+                    const uint4 r = philox4x32_10(make_uint4(0u, kTagWalk, plo, phi), P.seed, P.stream);
+                    st = M::neglog(u01<R>(r.x));
+                    u_roul = u01<R>(r.y); u_th = u01<R>(r.z); u_phi = u01<R>(r.w);
+                    alive = true;
+                    ++n_pk;
+                }
+                b_next += (uint32_t)__popcll(need) < avail ? (uint32_t)__popcll(need) : avail;
+            }
+        }
+        if (__ballot(alive) == 0ull) break;
+        if (!alive) continue;
+
+        // ------------------------------------------------------------ one event
+        bool ended = false;      // packet left through an end voxel
+        bool killed = false;     // packet ended by roulette / event guard
+        int xk = 0, xs = 1;      // axis and sign of the final crossing
+        if (lab == end) {
+            // launched inside an end voxel: leaves at once
+            ended = true; xk = 2; xs = vz > (R)0 ? 1 : -1;
+        } else {
+            ++nev;
+            // distance to the exit face, x < y < z on ties (_kernel.py:63-68)
+            const R d0 = (half - (vx > (R)0 ? px : -px)) * ivx;
+            const R d1 = (half - (vy > (R)0 ? py : -py)) * ivy;
+            const R d2 = (half - (vz > (R)0 ? pz : -pz)) * ivz;
+            R db = (R)1000;
+            int k = 0;
+            if (d0 < db) { db = d0; k = 0; }
+            if (d1 < db) { db = d1; k = 1; }
+            if (d2 < db) { db = d2; k = 2; }
+            const R smt = db * mt;
+            if (st >= smt) {
+                // ------------------------------------------------ face crossing (:69-120)
+                ++n_cross;
+                px += vx * db; py += vy * db; pz += vz * db;
+                const R vk = k == 0 ? vx : (k == 1 ? vy : vz);
+                const R face = M::copysign(half, vk);
+                if (k == 0) px = face; else if (k == 1) py = face; else pz = face;
+                st -= smt;
+                const int s = vk > (R)0 ? 1 : -1;
+                const uint32_t stride = k == 0 ? P.sx : (k == 1 ? P.sy : 1u);
+                const uint32_t nvi = s > 0 ? vi + stride : vi - stride;
+                uint32_t nxt = nvi < P.nvox ? (uint32_t)P.labels[nvi] : end;
+                nxt = nxt < end ? nxt : end;
+                const R nt = tab[nxt].n;
+                bool transmit = true;
+                if (ni != nt) {
+                    // Fresnel (:82-108) in cosine form; equal to the acos/asin/sin/tan form
+                    R u;
+                    if ((fres & 3u) == 0u) {
+                        const uint4 r = philox4x32_10(make_uint4(fres >> 2, kTagFresnel, (uint32_t)pid,
+                                                                 (uint32_t)(pid >> 32)), P.seed, P.stream);
+                        u = u01<R>(r.x); fb0 = u01<R>(r.y); fb1 = u01<R>(r.z); fb2 = u01<R>(r.w);
+                    } else {
+                        u = fb0; fb0 = fb1; fb1 = fb2;
+                    }
+                    ++fres;
+                    const R ci = M::fabs(vk);
+                    R rf, ct = (R)1;
+                    if (ci == (R)1) {
+                        // normal incidence: the reference takes R = 0 (ai == 0 branch, :92-93)
+                        const R q = (ni - nt) / (ni + nt);
+                        rf = normal_limit ? q * q : (R)0;
+                    } else {
+                        const R si = M::sqrt(M::fmax((R)1 - ci * ci, (R)0));
+                        const R sa = ni / nt * si;
+                        if (ni > nt && sa >= (R)1) {
+                            rf = (R)1;  // total internal reflection (:85-86)
+                        } else {
+                            ct = M::sqrt(M::fmax((R)1 - sa * sa, (R)0));
+                            const R sam = si * ct - ci * sa;   // sin(ai - at)
+                            const R sap = si * ct + ci * sa;   // sin(ai + at)
+                            const R cam = ci * ct + si * sa;   // cos(ai - at)
+                            const R cap = ci * ct - si * sa;   // cos(ai + at)
+                            const R q1 = M::div(sam, sap);
+                            const R q2 = M::div(cap, cam);
+                            rf = (R)0.5 * q1 * q1 * ((R)1 + q2 * q2);
+                        }
+                    }
+                    if (rf < u) {
+                        // refraction (:97-106)
+                        const R rn = ni / nt;
+                        R tx = vx * rn, ty = vy * rn, tz = vz * rn;
+                        const R nk = M::copysign(ct, vk);
+                        if (k == 0) tx = nk; else if (k == 1) ty = nk; else tz = nk;
+                        const R inv = M::rsqrt(tx * tx + ty * ty + tz * tz);
+                        vx = tx * inv; vy = ty * inv; vz = tz * inv;
+                        ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
+                    } else {
+                        transmit = false;
+                    }
+                }
+                if (transmit) {
+                    vi = nvi;
+                    if (k == 0) px = -px; else if (k == 1) py = -py; else pz = -pz;
+                    lab = nxt;
+                    if (lab == end) {
+                        ended = true; xk = k; xs = s;
+                    } else {
+                        mt = tab[lab].mt;
+                        ni = nt;
+                    }
+                } else {
+                    if (k == 0) vx = -vx; else if (k == 1) vy = -vy; else vz = -vz;  // (:119-120)
+                }
+            } else {
+                // ------------------------------------------------ hop ends in the voxel
+                const R sm = M::div(st, mt);
+                px += vx * sm; py += vy * sm; pz += vz * sm;       // (:122-124)
+                const LabelConst<R>& L = tab[lab];
+                const R wn = w - w * L.ra;                         // drop (:130)
+                const double dep = (double)w - (double)wn;        // exact in FP64
+                w = wn;
+                acc_abs += dep;
+                if (P.absorbed) {
+                    if (vi == dep_vi) {
+                        dep_acc += dep;
+                    } else {
+                        if (dep_vi != 0xffffffffu) gmem_add(P.absorbed + dep_vi, dep_acc);
+                        dep_vi = vi;
+                        dep_acc = dep;
+                    }
+                }
+                if (w <= (R)P.wth) {                               // roulette (:132-141)
+                    lds_add(&bev[kBE_ROULETTE], 1ull);
+                    if ((R)P.inv_m < u_roul) {
+                        killed = true;
+                    } else {
+                        const R wm = w * (R)P.roulette_m;
+                        lds_add(&bscal[kBS_GAIN], (double)wm - (double)w);
+                        w = wm;
+                    }
+                }
+                if (!killed) {
+                    // Henyey-Greenstein spin (:143-165)
+                    R c;
+                    if (L.g != (R)0) {
+                        const R t = L.hg2 * M::rcp(L.hg3 + L.hg4 * u_th);
+                        c = (L.hg1 - t * t) * L.hginv;
+                        c = c < (R)-1 ? (R)-1 : (c > (R)1 ? (R)1 : c);
+                    } else {
+                        c = (R)2 * u_th - (R)1;
+                    }
+                    const R sn = M::sqrt(M::fmax((R)1 - c * c, (R)0));
+                    R sf, cf;
+                    M::sincos2pi(u_phi, &sf, &cf);
+                    R nx_, ny_, nz_;
+                    if (M::fabs(vz) > (R)0.99999) {
+                        nx_ = sn * cf;
+                        ny_ = sn * sf;
+                        nz_ = vz > (R)0 ? c : -c;              // sign(v_z) * cos(theta), :153
+                    } else {
+                        const R den = M::sqrt((R)1 - vz * vz);
+                        const R iden = M::rcp(den);
+                        nx_ = sn * (vx * vz * cf - vy * sf) * iden + vx * c;
+                        ny_ = sn * (vy * vz * cf + vx * sf) * iden + vy * c;
+                        nz_ = -sn * cf * den + vz * c;
+                    }
+                    const R inv = M::rsqrt(nx_ * nx_ + ny_ * ny_ + nz_ * nz_);
+                    vx = nx_ * inv; vy = ny_ * inv; vz = nz_ * inv;
+                    ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
+                    ++inter;
+                    const uint4 r = philox4x32_10(make_uint4(inter, kTagWalk, (uint32_t)pid,
+                                                             (uint32_t)(pid >> 32)), P.seed, P.stream);
+                    st = M::neglog(u01<R>(r.x));
+                    u_roul = u01<R>(r.y); u_th = u01<R>(r.z); u_phi = u01<R>(r.w);
+                } else {
+                    ++inter;
+                }
+            }
+            if (!ended && !killed && nev >= P.max_events) killed = true;  // runaway guard
+        }
+
+        if (ended || killed) {
+            // ------------------------------------------------------ packet end
+            lds_add(&bev[kBE_INTERACT], (unsigned long long)inter);
+            lds_add(&bev[kBE_FRESNEL], (unsigned long long)fres);
+            const uint64_t slot = pid - P.first_packet;
+            const uint32_t az = vi % P.nz;
+            const uint32_t t2 = vi / P.nz;
+            const uint32_t ay = t2 % P.ny;
+            const uint32_t ax = t2 / P.ny;
+            if (killed) {
+                lds_add(&bev[kBE_KILLED], 1ull);
+                lds_add(&bscal[kBS_KILL], (double)w);
+                if (P.exit_state) {
+                    const unsigned long long n = P.n_packets;
+                    float* e = P.exit_state;
+                    e[slot] = -P.pitch * (float)P.cx;
+                    e[n + slot] = -P.pitch * (float)P.cy;
+                    e[2 * n + slot] = -0.5f * P.pitch;
+                    e[3 * n + slot] = 0.f; e[4 * n + slot] = 0.f; e[5 * n + slot] = 0.f;
+                    e[6 * n + slot] = 0.f;
+                }
+            } else {
+                lds_add(&bev[kBE_ESCAPED], 1ull);
+                const double wd = (double)w;
+                // exit encoding, voxel/_classes.py:261-269
+                const float x = P.pitch * (float)((int32_t)ax - P.cx) + (float)px;
+                const float y = P.pitch * (float)((int32_t)ay - P.cy) + (float)py;
+                const float z = P.pitch * (float)((int32_t)az - 1) + (float)pz + 0.5f * P.pitch;
+                const float fvz = (float)vz;
+                lds_add(&bscal[fvz < 0.f ? kBS_RD : (fvz > 0.f ? kBS_TT : kBS_OTHER)], wd);
+                lds_add(&bscal[kBS_FACE + 2 * xk + (xs > 0 ? 1 : 0)], wd);
+                const bool top = (xk == 2 && xs < 0);
+                const float r = __builtin_sqrtf(x * x + y * y);
+                if (P.nr > 0) {
+                    const int b = bin_of(r, P.dr, P.nr);
+                    if (b >= 0) {
+                        if (fvz < 0.f && P.rd_r) lds_add(&hist[P.h_rd + b], wd);
+                        if (fvz > 0.f && P.tt_r) lds_add(&hist[P.h_tt + b], wd);
+                        if (top && P.rd_top_r) lds_add(&hist[P.h_top + b], wd);
+                    }
+                }
+                if (P.na > 0) {
+                    const float ang = acosf(__builtin_fabsf(fvz));
+                    const int b = bin_of(ang, P.da, P.na);
+                    if (b >= 0) {
+                        if (fvz < 0.f && P.rd_a) lds_add(&hist[P.h_rda + b], wd);
+                        if (fvz > 0.f && P.tt_a) lds_add(&hist[P.h_tta + b], wd);
+                    }
+                }
+                if (P.nl > 0 && P.line_z && (float)vx < 0.f && __builtin_fabsf(y) < P.line_y) {
+                    const int b = bin_of(z, P.dl, P.nl);
+                    if (b >= 0) lds_add(&hist[P.h_line + b], wd);
+                }
+                if (top && P.n_rings > 0 && P.ring_w) {
+                    for (int i = 0; i < P.n_rings; ++i)
+                        if (r > P.rings[2 * i] && r <= P.rings[2 * i + 1]) lds_add(&hist[P.h_ring + i], wd);
+                }
+                if (P.exit_state) {
+                    const unsigned long long n = P.n_packets;
+                    float* e = P.exit_state;
+                    e[slot] = x; e[n + slot] = y; e[2 * n + slot] = z;
+                    e[3 * n + slot] = (float)vx; e[4 * n + slot] = (float)vy; e[5 * n + slot] = fvz;
+                    e[6 * n + slot] = (float)w;
+                }
+            }
+            alive = false;
+        }
+    }
+
+    // ---------------------------------------------------------------- block flush
+    if (P.absorbed && dep_vi != 0xffffffffu) gmem_add(P.absorbed + dep_vi, dep_acc);
+    lds_add(&bscal[kBS_ABS], acc_abs);
+    lds_add(&bscal[kBS_W0], (double)n_pk * (double)P.w0);
+    lds_add(&bev[kBE_CROSS], (unsigned long long)n_cross);
+    lds_add(&bev[kBE_PACKETS], (unsigned long long)n_pk);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kBS_COUNT; i += blockDim.x)
+        if (bscal[i] != 0.0) gmem_add(P.scalars + i, bscal[i]);
+    if (P.events)
+        for (uint32_t i = threadIdx.x; i < kBE_COUNT; i += blockDim.x)
+            if (bev[i] != 0ull) atomicAdd(P.events + i, bev[i]);
+    for (uint32_t i = threadIdx.x; i < P.h_total; i += blockDim.x) {
+        const double v = hist[i];
+        if (v == 0.0) continue;
+        double* dst;
+        if (i < P.h_tt) dst = P.rd_r + (i - P.h_rd);
+        else if (i < P.h_top) dst = P.tt_r + (i - P.h_tt);
+        else if (i < P.h_rda) dst = P.rd_top_r + (i - P.h_top);
+        else if (i < P.h_tta) dst = P.rd_a + (i - P.h_rda);
+        else if (i < P.h_line) dst = P.tt_a + (i - P.h_tta);
+        else if (i < P.h_ring) dst = P.line_z + (i - P.h_line);
+        else dst = P.ring_w + (i - P.h_ring);
+        gmem_add(dst, v);
+    }
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int n)
+{
+    const int i = threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ void philox_kernel(const uint4* ctr, uint32_t k0, uint32_t k1, uint4* out, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = philox4x32_10(ctr[i], k0, k1);
+}
+
+}  // namespace pmo

@@ -1,0 +1,186 @@
+"""Host runtime of the HIP walk: device buffers (torch tensors used as plain device
+memory), tally sets and the call into libpmo.so on torch's current HIP stream.
+
+`Walker.walk` is the one product entry to the hot path; the model classes in voxel.py,
+the batched sweep and the adjoint are built on it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _abi
+from .geometry import Source, validate_volume
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+@dataclass
+class TallySpec:
+    """Which exit/volume tallies a walk accumulates (sizes fix the LDS layout)."""
+    nr: int = 0
+    dr: float = 0.1
+    radial: tuple = ("rd_r", "tt_r", "rd_top_r")
+    na: int = 0
+    nl: int = 0
+    dl: float = 0.1
+    line_y: float = 5.0
+    rings: tuple = ()            # ((r_in, r_out), ...) mm, top-face annuli
+    absorbed: bool = True
+    exit_state: bool = False
+    events: bool = True
+
+
+class TallySet:
+    """FP64 device tally buffers for one volume; zero() before reuse."""
+
+    def __init__(self, spec: TallySpec, nvox: int, n_packets: int, device):
+        self.spec = spec
+        f64 = dict(dtype=torch.float64, device=device)
+        self.scalars = torch.zeros(_abi.PMO_S_COUNT, **f64)
+        self.events = torch.zeros(_abi.PMO_EV_COUNT, dtype=torch.int64, device=device) if spec.events else None
+        self.absorbed = torch.zeros(nvox, **f64) if spec.absorbed else None
+        self.hist = {}
+        if spec.nr > 0:
+            for k in spec.radial:
+                self.hist[k] = torch.zeros(spec.nr, **f64)
+        if spec.na > 0:
+            self.hist["rd_a"] = torch.zeros(spec.na, **f64)
+            self.hist["tt_a"] = torch.zeros(spec.na, **f64)
+        if spec.nl > 0:
+            self.hist["line_z"] = torch.zeros(spec.nl, **f64)
+        self.rings = None
+        if spec.rings:
+            r = np.asarray(spec.rings, np.float32).reshape(-1)
+            self.rings = torch.from_numpy(r).to(device)
+            self.hist["ring_w"] = torch.zeros(r.size // 2, **f64)
+        self.exit_state = (torch.zeros(7 * int(n_packets), dtype=torch.float32, device=device)
+                           if spec.exit_state else None)
+        self.n_exit = int(n_packets) if spec.exit_state else 0
+
+    def zero(self):
+        self.scalars.zero_()
+        if self.events is not None:
+            self.events.zero_()
+        if self.absorbed is not None:
+            self.absorbed.zero_()
+        for v in self.hist.values():
+            v.zero_()
+        if self.exit_state is not None:
+            self.exit_state.zero_()
+
+    def c_struct(self) -> _abi.PmoTallies:
+        s = self.spec
+        t = _abi.PmoTallies()
+        t.scalars = _ptr(self.scalars)
+        t.events = _ptr(self.events)
+        t.absorbed = _ptr(self.absorbed)
+        t.nr, t.dr = int(s.nr), float(s.dr)
+        t.rd_r, t.tt_r, t.rd_top_r = (_ptr(self.hist.get(k)) for k in ("rd_r", "tt_r", "rd_top_r"))
+        t.na = int(s.na)
+        t.rd_a, t.tt_a = _ptr(self.hist.get("rd_a")), _ptr(self.hist.get("tt_a"))
+        t.nl, t.dl, t.line_y = int(s.nl), float(s.dl), float(s.line_y)
+        t.line_z = _ptr(self.hist.get("line_z"))
+        t.n_rings = 0 if self.rings is None else int(self.rings.numel() // 2)
+        t.rings = _ptr(self.rings)
+        t.ring_w = _ptr(self.hist.get("ring_w"))
+        t.exit_state = _ptr(self.exit_state)
+        return t
+
+    def to_host(self, shape=None) -> dict:
+        """Raw (unnormalised) sums, same layout as the oracle's walk()."""
+        out = {"scalars": dict(zip(_abi.SCALAR_NAMES, self.scalars.cpu().tolist()))}
+        if self.events is not None:
+            out["events"] = dict(zip(_abi.EVENT_NAMES, [int(x) for x in self.events.cpu().tolist()]))
+        if self.absorbed is not None:
+            a = self.absorbed.cpu().numpy()
+            out["absorbed"] = a.reshape(shape) if shape is not None else a
+        for k, v in self.hist.items():
+            out[k] = v.cpu().numpy()
+        if self.exit_state is not None:
+            out["exit_state"] = self.exit_state.cpu().numpy().reshape(7, self.n_exit)
+        return out
+
+
+class Walker:
+    """A label volume and optics table resident on the GPU, ready for walks."""
+
+    def __init__(self, labels: np.ndarray, pitch: float, optics: np.ndarray, device="cuda:0",
+                 validate=True):
+        labels = np.ascontiguousarray(labels)
+        optics = np.ascontiguousarray(optics, dtype=np.float32)
+        if optics.ndim != 2 or optics.shape[1] != 4:
+            raise ValueError("optics must be float32 [n_labels+1][4]")
+        self.n_labels = optics.shape[0] - 1
+        if validate:
+            validate_volume(labels, self.n_labels)
+        self.shape = tuple(int(s) for s in labels.shape)
+        self.pitch = float(pitch)
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise RuntimeError("the walk runs on a HIP device only (no CPU fallback)")
+        self.lib = _abi.lib()
+        self.labels = torch.from_numpy(labels.reshape(-1)).to(self.device)
+        self.optics = torch.from_numpy(optics).to(self.device)
+        self.workspace = torch.zeros(int(self.lib.pmo_workspace_bytes()), dtype=torch.uint8,
+                                     device=self.device)
+
+    @property
+    def nvox(self):
+        return self.shape[0] * self.shape[1] * self.shape[2]
+
+    def set_optics(self, optics: np.ndarray):
+        o = torch.as_tensor(np.ascontiguousarray(optics, dtype=np.float32))
+        if tuple(o.shape) != tuple(self.optics.shape):
+            raise ValueError("optics table shape cannot change")
+        self.optics.copy_(o)
+
+    def tallies(self, spec: TallySpec, n_packets: int = 0) -> TallySet:
+        return TallySet(spec, self.nvox, n_packets, self.device)
+
+    def volume_struct(self) -> _abi.PmoVolume:
+        v = _abi.PmoVolume()
+        v.labels = _ptr(self.labels)
+        v.nx, v.ny, v.nz = self.shape
+        v.n_labels = self.n_labels
+        v.pitch = self.pitch
+        v.optics = _ptr(self.optics)
+        return v
+
+    def walk(self, src: Source, n_packets: int, seed: int, tallies: TallySet, *, stream_id=0,
+             first_packet=0, wth=1e-4, roulette_m=10.0, fp64=False, normal_fresnel=False,
+             block_size=0, grid_blocks=0, packets_per_lane=0, hip_stream=None):
+        """Launch one walk (asynchronous on torch's current stream unless given)."""
+        if tallies.exit_state is not None and tallies.n_exit != int(n_packets):
+            raise ValueError("exit_state buffer was sized for a different packet count")
+        vol = self.volume_struct()
+        s = _abi.PmoSource()
+        s.kind = int(src.kind)
+        s.ix, s.iy, s.iz = (int(x) for x in src.voxel)
+        s.radius = float(src.radius)
+        for i in range(3):
+            s.dir[i] = float(src.dir[i])
+        s.w0 = float(src.w0)
+        r = _abi.PmoRun()
+        r.n_packets = int(n_packets)
+        r.first_packet = int(first_packet)
+        r.seed = int(seed) & 0xFFFFFFFF
+        r.stream = int(stream_id) & 0xFFFFFFFF
+        r.wth = float(wth)
+        r.roulette_m = float(roulette_m)
+        r.flags = (_abi.PMO_F_FP64 if fp64 else 0) | (_abi.PMO_F_NORMAL_FRESNEL if normal_fresnel else 0)
+        r.block_size = int(block_size)
+        r.grid_blocks = int(grid_blocks)
+        r.packets_per_lane = int(packets_per_lane)
+        t = tallies.c_struct()
+        if hip_stream is None:
+            hip_stream = torch.cuda.current_stream(self.device).cuda_stream
+        with torch.cuda.device(self.device):
+            rc = self.lib.pmo_walk(C.byref(vol), C.byref(s), C.byref(r), C.byref(t),
+                                   _ptr(self.workspace), C.c_void_p(hip_stream))
+        _abi.check(rc)

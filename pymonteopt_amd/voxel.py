@@ -1,0 +1,408 @@
+"""pymopt voxel-model API on the MI355X walk.
+
+Mirrors pymopt/voxel_gpu/_classes.py (and the CPU twin pymopt/voxel/_classes.py):
+`VoxelPlateModel(nPh=...).build(**params).start(rand_seed).get_result()`, with the same
+parameter keys, the same KeyError contract of set_params (pymopt/utils/utilities.py:22-32)
+and the same launch conventions. The walk itself runs in libpmo.so; the per-packet
+post-processing of the reference (getRdTtRate, radialDistance, the virtualOBD detectors)
+is replaced by FP64 tallies accumulated on the device, and get_result() returns those.
+"""
+from __future__ import annotations
+
+import bz2
+import datetime
+import json
+import pickle
+import time
+
+import numpy as np
+import torch
+
+from . import _abi
+from .geometry import (Source, center_add, fresnel_oblique, optics_table, plate_volume, specular,
+                       validate_volume)
+from .runtime import TallySpec, Walker
+
+__all__ = ["set_params", "PlateModel", "LabelVolumeModel", "BaseVoxelMonteCarlo", "VoxelPlateModel",
+           "VoxelLabelModel", "normalise"]
+
+
+def set_params(data, keys, *initial_data, **kwargs):
+    """Update `data` from dicts and keywords; unknown keys raise KeyError
+    (pymopt/utils/utilities.py:22-32)."""
+    for dictionary in initial_data:
+        for key in dictionary:
+            if key not in keys:
+                raise KeyError(key)
+            data[key] = dictionary[key]
+    for key in kwargs:
+        if key not in keys:
+            raise KeyError(key)
+        data[key] = kwargs[key]
+
+
+class VoxelModel:
+    def build(self):
+        pass
+
+    def set_params(self):
+        pass
+
+    def getModelSize(self):
+        print("Memory area size for voxel storage: %0.3f Mbyte" % (self.voxel_model.nbytes * 1e-6))
+
+
+class PlateModel(VoxelModel):
+    """Layered plate (pymopt/voxel_gpu/_classes.py:396-468, pymopt/voxel/_classes.py:352-424)."""
+
+    def __init__(self):
+        self.model_name = "PlateModel"
+        self.dtype_f = np.float32
+        self.dtype = np.uint8
+        self.params = {
+            "x_size": 40, "y_size": 40,
+            "voxel_space": 0.1,
+            "thickness": [1.3, 40],
+            "n": [1.5, 1.4],
+            "n_air": 1.,
+            "ma": [1e-5, 0.02374],
+            "ms": [1e-5, 0.02374],
+            "g": [0.9, 0.9],
+        }
+        self.keys = list(self.params.keys())
+        self._param_instantiating()
+        self.voxel_model = np.zeros((3, 3, 3), dtype=self.dtype)
+
+    def _param_instantiating(self):
+        f = self.dtype_f
+        self.thickness = self.params["thickness"]
+        self.n = np.array(list(self.params["n"]) + [self.params["n_air"]]).astype(f)
+        self.ms = np.array(self.params["ms"]).astype(f)
+        self.ma = np.array(self.params["ma"]).astype(f)
+        self.g = np.array(self.params["g"]).astype(f)
+        self.voxel_space = self.params["voxel_space"]
+        self.x_size = np.int32(round(self.params["x_size"] / self.params["voxel_space"]))
+        self.y_size = np.int32(round(self.params["y_size"] / self.params["voxel_space"]))
+        self.z_size = np.int32(round(np.array(self.thickness).sum() / self.params["voxel_space"]))
+        self.end_point = len(self.thickness)
+
+    def build(self):
+        self.voxel_model = plate_volume(self.thickness, self.voxel_space, self.params["x_size"],
+                                        self.params["y_size"])
+        self.end_point = len(self.thickness)
+        self.getModelSize()
+
+    def set_params(self, *initial_data, **kwargs):
+        set_params(self.params, self.keys, *initial_data, **kwargs)
+        self._param_instantiating()
+
+    def get_params(self):
+        return {"th": self.thickness, "ms": self.ms, "ma": self.ma, "n": self.n, "g": self.g}
+
+    def optics(self):
+        return optics_table(self.params["ma"], self.params["ms"], self.params["g"], self.params["n"],
+                            self.params["n_air"])
+
+    @property
+    def n_ambient(self):
+        return float(self.params["n_air"])
+
+    @property
+    def n_first(self):
+        return float(self.params["n"][0])
+
+
+class LabelVolumeModel(VoxelModel):
+    """Arbitrary label volume with a per-label optics table (the role of the reference's
+    TuringModel_* classes, pymopt/voxel_gpu/_classes.py:554-821, for volumes built
+    elsewhere). `build(volume)` takes the padded uint8 volume whose shell is the end label;
+    the beam enters at the top-face centre like the plate's."""
+
+    def __init__(self):
+        self.model_name = "LabelVolumeModel"
+        self.params = {"voxel_space": 0.02, "ma": [0.1], "ms": [10.0], "g": [0.9], "n": [1.37],
+                       "n_air": 1.0, "launch_z": 1}
+        self.keys = list(self.params.keys())
+        self.voxel_model = np.zeros((3, 3, 3), dtype=np.uint8)
+        self._param_instantiating()
+
+    def _param_instantiating(self):
+        self.voxel_space = self.params["voxel_space"]
+        self.n = np.array(list(self.params["n"]) + [self.params["n_air"]], np.float32)
+        self.end_point = len(self.params["ma"])
+
+    def set_params(self, *initial_data, **kwargs):
+        set_params(self.params, self.keys, *initial_data, **kwargs)
+        self._param_instantiating()
+
+    def build(self, volume=None):
+        if volume is not None:
+            self.voxel_model = np.ascontiguousarray(volume, dtype=np.uint8)
+        validate_volume(self.voxel_model, self.end_point)
+        self.getModelSize()
+
+    def get_params(self):
+        return {k: self.params[k] for k in ("ma", "ms", "g", "n")}
+
+    def optics(self):
+        return optics_table(self.params["ma"], self.params["ms"], self.params["g"], self.params["n"],
+                            self.params["n_air"])
+
+    @property
+    def n_ambient(self):
+        return float(self.params["n_air"])
+
+    @property
+    def n_first(self):
+        vm = self.voxel_model
+        lab = int(vm[center_add(vm.shape[0]), center_add(vm.shape[1]), int(self.params["launch_z"])])
+        return float(self.params["n"][lab])
+
+
+def normalise(raw: dict, n_packets: int, spec: TallySpec, pitch: float, optics: np.ndarray,
+              shape=None, interior=True) -> dict:
+    """Turn raw FP64 sums into the reference's normalised quantities
+    (getRdTtRate, radialDistance, angularyResolved, lineDistance; pymopt/metrics/_scoring.py)."""
+    N = float(n_packets)
+    s = raw["scalars"]
+    out = {"nPh": int(n_packets), "Rd": s["Rd"] / N, "Tt": s["Tt"] / N, "A": s["A"] / N,
+           "energy": {k: v / N for k, v in s.items()}}
+    if "events" in raw:
+        out["events"] = raw["events"]
+    if spec.nr > 0:
+        dr = spec.dr
+        alpha = np.array([i * dr for i in range(spec.nr + 1)])
+        da = np.array([2 * np.pi * (i + 0.5) * dr ** 2 for i in range(spec.nr)])
+        out["r"] = alpha[:-1]
+        for k, name in (("rd_r", "Rd_r"), ("tt_r", "Tt_r"), ("rd_top_r", "Rd_top_r")):
+            if k in raw:
+                out[name] = raw[k] / (da * N)
+    if spec.na > 0:
+        d = np.pi / (2 * spec.na)
+        alpha = np.array([(i + 0.5) * d for i in range(spec.na + 1)])
+        do = 4 * np.pi * np.sin(alpha) * np.sin(d / 2)
+        out["alpha"] = alpha[:-1]
+        out["Rd_a"] = raw["rd_a"] / (do[:-1] * N)
+        out["Tt_a"] = raw["tt_a"] / (do[:-1] * N)
+    if spec.nl > 0:
+        out["z"] = np.array([i * spec.dl for i in range(spec.nl)])
+        out["line_z"] = raw["line_z"] / (spec.dl * spec.line_y * 2 * N)
+    if spec.rings:
+        rings = np.asarray(spec.rings, np.float64).reshape(-1, 2)
+        area = np.pi * (rings[:, 1] ** 2 - rings[:, 0] ** 2)
+        out["rings"] = rings
+        out["ring_R"] = raw["ring_w"] / (area * N)
+    if "absorbed" in raw:
+        a = raw["absorbed"]
+        if shape is not None:
+            a = a.reshape(shape)
+        if interior:
+            a = a[1:-1, 1:-1, 1:-1]
+        out["absorbed"] = a / N
+        labels = None
+        out["_absorbed_raw_interior"] = a
+    return out
+
+
+class BaseVoxelMonteCarlo:
+    """Walk driver with the reference's constructor/start/get_result surface
+    (pymopt/voxel_gpu/_classes.py:41-382)."""
+
+    def __init__(self, *, nPh, model, dtype_f=np.float32, dtype=np.int32, beam_type="TEM00", w_beam=0,
+                 beam_angle=0, initial_refrect_by_angle=False, first_layer_clear=False, threadnum=128,
+                 nr=50, dr=0.1, na=0, nl=0, dl=0.1, line_y=5.0, rings=(), fluence=True,
+                 exit_state=False, fp64=False, normal_fresnel=False, device="cuda:0"):
+        beam_type_list = ["TEM00", "flat", False]
+        if beam_type not in beam_type_list:
+            raise ValueError("%s is not a permitted for factor. Please choose from %s." % (beam_type, beam_type_list))
+        if threadnum > 1024:
+            raise Exception("There are too many threads. threadnum < 1024.")
+        if initial_refrect_by_angle:
+            raise NotImplementedError("initial_refrect_by_angle is not supported by the device walk")
+        if first_layer_clear:
+            raise NotImplementedError("first_layer_clear needs a PlateExModel geometry, not provided")
+        self.beam_type = beam_type
+        self.dtype = dtype
+        self.dtype_f = dtype_f
+        self.nPh = nPh
+        self.w_beam = w_beam
+        self.beam_angle = beam_angle
+        self.initial_refrect_by_angle = initial_refrect_by_angle
+        self.first_layer_clear = first_layer_clear
+        self.model = model
+        self.threadnum = int(threadnum)
+        self.spec = TallySpec(nr=int(nr), dr=float(dr), na=int(na), nl=int(nl), dl=float(dl),
+                              line_y=float(line_y), rings=tuple(tuple(r) for r in rings),
+                              absorbed=bool(fluence), exit_state=bool(exit_state))
+        self.fp64 = bool(fp64)
+        self.normal_fresnel = bool(normal_fresnel)
+        self.device = device
+        self._walker = None
+        self._result = None
+        self.time_s = None
+
+    # ----------------------------------------------------------------- set-up
+    def build(self, *initial_data, **kwargs):
+        if initial_data == () and kwargs == {}:
+            pass
+        else:
+            self.model.set_params(*initial_data, **kwargs)
+        self.model.build()
+        self._walker = None
+        return self
+
+    def set_monte_params(self, *, nPh, model, dtype_f=np.float32, dtype=np.int32, w_beam=0):
+        self.dtype = dtype
+        self.dtype_f = dtype_f
+        self.nPh = nPh
+        self.w_beam = w_beam
+        self.model = model
+        self._walker = None
+
+    def get_voxel_model(self):
+        return self.model.voxel_model
+
+    def get_model_params(self):
+        return self.model.get_params()
+
+    def source(self) -> Source:
+        """Launch description, pymopt/voxel/_classes.py:107-213."""
+        vm = self.model.voxel_model
+        ix, iy = center_add(vm.shape[0]), center_add(vm.shape[1])
+        iz = int(getattr(self.model, "params", {}).get("launch_z", 1))
+        n1, n2 = self.model.n_ambient, self.model.n_first
+        direction = (0.0, 0.0, 1.0)
+        rsp = specular(n1, n2)
+        if self.beam_angle != 0 and self.w_beam == 0:
+            at, r = fresnel_oblique(float(self.beam_angle), n1, n2)
+            direction = (float(np.sin(at)), 0.0, float(np.cos(at)))
+            rsp = r if n1 != n2 else 0.0
+        kind, radius = _abi.PMO_BEAM_PENCIL, 0.0
+        if self.beam_type == "flat" and self.w_beam != 0:
+            kind, radius = _abi.PMO_BEAM_FLAT, float(self.w_beam)
+        elif self.beam_type == "TEM00" and self.w_beam != 0:
+            kind, radius = _abi.PMO_BEAM_GAUSS, float(self.w_beam)
+        return Source(voxel=(ix, iy, iz), w0=float(np.float32(1.0) - np.float32(rsp)), kind=kind,
+                      radius=radius, dir=direction)
+
+    def walker(self) -> Walker:
+        if self._walker is None:
+            self._walker = Walker(self.model.voxel_model, float(self.model.voxel_space), self.model.optics(),
+                                  device=self.device)
+        return self._walker
+
+    # ----------------------------------------------------------------- run
+    def start(self, rand_seed=None):
+        if rand_seed is None:
+            rand_seed = np.random.randint(1e5)
+        self.nPh = int(self.nPh)
+        self.rand_seed = int(rand_seed)
+        wk = self.walker()
+        tal = wk.tallies(self.spec, self.nPh)
+        print("")
+        print("###### Start (Random seed: %s) ######" % rand_seed)
+        print("")
+        bs = self.threadnum if self.threadnum % 64 == 0 else 0
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.time()
+        ev0.record()
+        wk.walk(self.source(), self.nPh, self.rand_seed, tal, fp64=self.fp64,
+                normal_fresnel=self.normal_fresnel, block_size=bs)
+        ev1.record()
+        ev1.synchronize()
+        self.time_s = ev0.elapsed_time(ev1) * 1e-3
+        raw = tal.to_host(shape=wk.shape)
+        self._raw = raw
+        self._result = normalise(raw, self.nPh, self.spec, wk.pitch, self.model.optics(), shape=wk.shape)
+        if "absorbed" in self._result:
+            self._result["fluence"] = self._fluence(self._result["_absorbed_raw_interior"])
+        self._result.pop("_absorbed_raw_interior", None)
+        if self.spec.exit_state:
+            es = raw["exit_state"]
+            self._result["p"] = es[0:3].copy()
+            self._result["v"] = es[3:6].copy()
+            self._result["w"] = es[6].copy()
+        print("###### End ######")
+        self.getRdTtRate()
+        print("Calculation time: %0.3f seconds (walk %0.3f s on the GPU)." % (time.time() - t0, self.time_s))
+        return self
+
+    def _fluence(self, absorbed_raw_interior):
+        """Phi[v] = A[v] / (mu_a[label v] * l^3 * nPh) in mm^-2 (zero where mu_a = 0)."""
+        vm = self.model.voxel_model[1:-1, 1:-1, 1:-1]
+        opt = self.model.optics()
+        mua = opt[:, 0].astype(np.float64)[np.minimum(vm, opt.shape[0] - 1)]
+        l3 = float(self.model.voxel_space) ** 3
+        with np.errstate(divide="ignore", invalid="ignore"):
+            phi = np.where(mua > 0, absorbed_raw_interior / (mua * l3 * self.nPh), 0.0)
+        return phi
+
+    # ----------------------------------------------------------------- results
+    def get_result(self):
+        if self._result is None:
+            raise RuntimeError("start() has not been run")
+        return self._result
+
+    def getRdTtRate(self):
+        res = self.get_result()
+        self.Rdw = res["Rd"]
+        self.Ttw = res["Tt"]
+        print("######")
+        print("Mean Rd %0.6f" % self.Rdw)
+        print("Mean Td %0.6f" % self.Ttw)
+        print()
+
+    def getRdTtValues(self):
+        return {"Rd": self.Rdw, "Td": self.Ttw}
+
+    def save_result(self, fname, *, coment="", save_monte=True, save_params=True):
+        """Writes <fname>_LID.pkl.bz2 (the get_result dict) and <fname>_info.json
+        (pymopt/voxel/_classes.py:295-338)."""
+        if save_monte:
+            with bz2.open(fname + "_LID.pkl.bz2", "wb") as fp:
+                fp.write(pickle.dumps(self.get_result()))
+        if save_params:
+            info = {
+                "Date": datetime.datetime.now().isoformat(),
+                "coment": coment,
+                "number_of_photons": self.nPh,
+                "calc_dtype": "64 bit" if self.fp64 else "32 bit",
+                "model": {"model_name": self.model.model_name,
+                          "model_params": {k: np.asarray(v).tolist() for k, v in self.model.get_params().items()},
+                          "model_voxel_space": self.model.voxel_space},
+                "w_beam": self.w_beam,
+                "beam_angle": self.beam_angle,
+                "beam_mode": self.beam_type,
+                "rand_seed": getattr(self, "rand_seed", None),
+            }
+            with open(fname + "_info.json", "w") as fp:
+                json.dump(info, fp, indent=4)
+
+
+class VoxelPlateModel(BaseVoxelMonteCarlo):
+    """pymopt/voxel_gpu/_classes.py:827-841."""
+
+    def __init__(self, *, nPh=1000, dtype_f=np.float32, dtype=np.int32, beam_type="TEM00", w_beam=0,
+                 beam_angle=0, initial_refrect_by_angle=False, first_layer_clear=False, threadnum=128,
+                 **tally_kwargs):
+        super().__init__(nPh=nPh, model=PlateModel(), dtype_f=dtype_f, dtype=dtype, w_beam=w_beam,
+                         beam_angle=beam_angle, beam_type=beam_type,
+                         initial_refrect_by_angle=initial_refrect_by_angle,
+                         first_layer_clear=first_layer_clear, threadnum=threadnum, **tally_kwargs)
+
+
+class VoxelLabelModel(BaseVoxelMonteCarlo):
+    """Walk through an arbitrary label volume (the reference's VoxelTuringModel role,
+    pymopt/voxel_gpu/_classes.py:888-1006)."""
+
+    def __init__(self, *, nPh=1000, beam_type="TEM00", w_beam=0, beam_angle=0, threadnum=128, **tally_kwargs):
+        super().__init__(nPh=nPh, model=LabelVolumeModel(), beam_type=beam_type, w_beam=w_beam,
+                         beam_angle=beam_angle, threadnum=threadnum, **tally_kwargs)
+
+    def build(self, volume=None, *initial_data, **kwargs):
+        if initial_data or kwargs:
+            self.model.set_params(*initial_data, **kwargs)
+        self.model.build(volume)
+        self._walker = None
+        return self

@@ -1,0 +1,102 @@
+"""The C ABI library: loads without a GPU, exports every symbol include/pmo.h declares,
+its ctypes mirror matches the C layout, and argument errors come back as codes."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from pymonteopt_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pmo.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pmo_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _abi.lib()
+    names = declared_functions()
+    assert "pmo_walk" in names and "pmo_philox" in names
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in _abi.SIGNATURES, n
+    assert L.pmo_abi_version() == 1
+    assert L.pmo_workspace_bytes() >= 8
+
+
+PROBE = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "pmo.h"
+#define S(t) printf(#t " %zu\n", sizeof(t));
+#define O(t, f) printf(#t "." #f " %zu\n", offsetof(t, f));
+int main(void) {
+  S(pmo_volume) S(pmo_source) S(pmo_run) S(pmo_tallies)
+  O(pmo_volume, optics) O(pmo_source, w0) O(pmo_run, packets_per_lane) O(pmo_tallies, exit_state)
+  O(pmo_tallies, ring_w) O(pmo_tallies, line_z) O(pmo_run, flags)
+  printf("PMO_S_COUNT %d\nPMO_EV_COUNT %d\n", PMO_S_COUNT, PMO_EV_COUNT);
+  return 0;
+}
+"""
+
+
+def test_ctypes_layout_matches_header(tmp_path):
+    src = tmp_path / "probe.c"
+    exe = tmp_path / "probe"
+    src.write_text(PROBE)
+    subprocess.run(["gcc", "-std=c11", f"-I{os.path.dirname(HEADER)}", str(src), "-o", str(exe)], check=True)
+    out = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                              check=True).stdout.strip().split("\n"))
+    for name, cls in (("pmo_volume", _abi.PmoVolume), ("pmo_source", _abi.PmoSource),
+                      ("pmo_run", _abi.PmoRun), ("pmo_tallies", _abi.PmoTallies)):
+        assert int(out[name]) == C.sizeof(cls), name
+    assert int(out["pmo_volume.optics"]) == _abi.PmoVolume.optics.offset
+    assert int(out["pmo_source.w0"]) == _abi.PmoSource.w0.offset
+    assert int(out["pmo_run.packets_per_lane"]) == _abi.PmoRun.packets_per_lane.offset
+    assert int(out["pmo_run.flags"]) == _abi.PmoRun.flags.offset
+    assert int(out["pmo_tallies.exit_state"]) == _abi.PmoTallies.exit_state.offset
+    assert int(out["pmo_tallies.ring_w"]) == _abi.PmoTallies.ring_w.offset
+    assert int(out["pmo_tallies.line_z"]) == _abi.PmoTallies.line_z.offset
+    assert int(out["PMO_S_COUNT"]) == _abi.PMO_S_COUNT
+    assert int(out["PMO_EV_COUNT"]) == _abi.PMO_EV_COUNT
+
+
+def _valid_structs():
+    v = _abi.PmoVolume(labels=1, nx=5, ny=5, nz=5, n_labels=1, pitch=0.1, optics=1)
+    s = _abi.PmoSource(kind=0, ix=2, iy=2, iz=1, radius=0.0, w0=1.0)
+    s.dir[2] = 1.0
+    r = _abi.PmoRun(n_packets=10, seed=1, wth=1e-4, roulette_m=10.0)
+    t = _abi.PmoTallies(scalars=1)
+    return v, s, r, t
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("labels", None, "labels"), ("n_labels", 0, "n_labels"), ("pitch", 0.0, "pitch"),
+    ("ix", 0, "launch voxel"), ("kind", 9, "beam kind"), ("block_size", 100, "block_size"),
+    ("roulette_m", 0.5, "roulette_m"), ("scalars", None, "scalars"), ("nx", 2, "extents"),
+])
+def test_argument_errors_are_codes(field, value, msg):
+    # every check below runs on the host before any HIP call
+    L = _abi.lib()
+    v, s, r, t = _valid_structs()
+    for obj in (v, s, r, t):
+        if field in [f[0] for f in obj._fields_]:
+            setattr(obj, field, value)
+    rc = L.pmo_walk(C.byref(v), C.byref(s), C.byref(r), C.byref(t), C.c_void_p(1), None)
+    assert rc in (_abi.PMO_ERR_INVALID, _abi.PMO_ERR_LABEL)
+    assert msg in L.pmo_last_error().decode()
+    with pytest.raises(ValueError):
+        _abi.check(rc)
+
+
+def test_philox_argument_errors():
+    L = _abi.lib()
+    assert L.pmo_philox(None, 0, 0, None, -1, None) == _abi.PMO_ERR_INVALID
+    assert L.pmo_philox(None, 0, 0, None, 0, None) == _abi.PMO_OK
+    assert L.pmo_philox(None, 0, 0, None, 4, None) == _abi.PMO_ERR_INVALID

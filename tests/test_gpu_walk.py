@@ -1,0 +1,207 @@
+"""GPU parity of the HIP walk against the oracle (FP64 C restatement of the reference
+walk). Statistical bar: 3 pooled standard errors over 10 seed batches; energy to 1e-6."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from _parity import binned_z, gpu_run, ledger_error, oracle_run, zscore
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch
+
+
+@pytest.fixture(scope="module")
+def slab(torch_cuda):
+    from pymonteopt_amd import presets
+    from pymonteopt_amd.runtime import Walker
+    wl = presets.slab_plate()
+    return wl, Walker(wl.labels, wl.pitch, wl.optics)
+
+
+def test_device_philox_bit_exact(torch_cuda):
+    import ctypes as C
+    from pymonteopt_amd import _abi
+    torch = torch_cuda
+    d = json.load(open(os.path.join(HERE, "golden", "philox_vectors.json")))
+    vec = d["vectors"] + d["random123_kat"]
+    for key in {tuple(v["key"]) for v in vec}:
+        sel = [v for v in vec if tuple(v["key"]) == key]
+        ctr = torch.tensor(np.array([v["ctr"] for v in sel], np.uint32).view(np.int32)).cuda()
+        out = torch.zeros_like(ctr)
+        rc = _abi.lib().pmo_philox(C.c_void_p(ctr.data_ptr()), key[0], key[1], C.c_void_p(out.data_ptr()),
+                                   len(sel), C.c_void_p(torch.cuda.current_stream().cuda_stream))
+        _abi.check(rc)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy().view(np.uint32)
+        assert got.tolist() == [v["out"] for v in sel]
+
+
+def test_slab_parity_same_seeds(slab, oracle):
+    """Slab 101x101x50 @0.1 mm, 1e5 packets x seeds 0-9, FP32 walk vs FP64 oracle."""
+    wl, wk = slab
+    N = 100000
+    G, O = [], []
+    for s in range(10):
+        g = gpu_run(wk, wl, N, s)
+        o = oracle_run(oracle, wl, N, s)
+        assert ledger_error(g["scalars"]) < 1e-6
+        assert g["events"]["packets"] == N
+        assert g["events"]["escaped"] + g["events"]["killed"] == N
+        G.append(g)
+        O.append(o)
+    for k in ("Rd", "Tt", "A"):
+        a = [g["scalars"][k] / N for g in G]
+        b = [o["scalars"][k] / N for o in O]
+        assert zscore(a, b) < 3, k
+        # common random numbers: the two walks follow the same draws, so the per-seed
+        # differences are far below the seed-to-seed spread
+        se = np.std(b, ddof=1)
+        assert np.max(np.abs(np.subtract(a, b))) < se, k
+    zr = binned_z([g["rd_r"] / N for g in G], [o["rd_r"] / N for o in O])
+    assert np.sum(zr > 3) <= 2 and np.all(zr < 5)
+    # absorbed weight per depth slice and per voxel
+    da = binned_z([g["absorbed"].sum(axis=(0, 1)) / N for g in G], [o["absorbed"].sum(axis=(0, 1)) / N for o in O])
+    assert np.sum(da > 3) <= 2 and np.all(da < 5)
+    ga = np.mean([g["absorbed"] for g in G], axis=0)
+    oa = np.mean([o["absorbed"] for o in O], axis=0)
+    assert np.abs(ga - oa).sum() / oa.sum() < 0.02
+    assert ga.sum() == pytest.approx(np.mean([g["scalars"]["A"] for g in G]), rel=1e-9)
+
+
+def test_slab_parity_independent_streams(slab, oracle):
+    """Same slab; GPU seeds 100-109 against oracle seeds 0-9 (independent streams)."""
+    wl, wk = slab
+    N = 100000
+    G = [gpu_run(wk, wl, N, 100 + s) for s in range(10)]
+    O = [oracle_run(oracle, wl, N, s) for s in range(10)]
+    for k in ("Rd", "Tt", "A"):
+        assert zscore([g["scalars"][k] for g in G], [o["scalars"][k] for o in O]) < 3, k
+    zr = binned_z([g["rd_r"] for g in G], [o["rd_r"] for o in O])
+    assert np.sum(zr > 3) <= 2 and np.all(zr < 5)
+
+
+def test_fp64_build_deviation(slab):
+    """The FP64 build of the kernel walks the same draws; FP32 deviates by rounding only."""
+    wl, wk = slab
+    N = 100000
+    a = [gpu_run(wk, wl, N, s) for s in range(5)]
+    b = [gpu_run(wk, wl, N, s, fp64=True) for s in range(5)]
+    for k in ("Rd", "Tt", "A"):
+        x = np.array([r["scalars"][k] / N for r in a])
+        y = np.array([r["scalars"][k] / N for r in b])
+        spread = np.std(y, ddof=1)
+        assert np.max(np.abs(x - y)) < 0.5 * spread, (k, x - y, spread)
+    assert ledger_error(b[0]["scalars"]) < 1e-12
+
+
+def test_exit_state_dump_matches_tallies(slab):
+    from pymonteopt_amd import metrics
+    from pymonteopt_amd.runtime import TallySpec
+    wl, wk = slab
+    N = 50000
+    spec = TallySpec(nr=50, dr=0.1, na=30, nl=40, dl=0.125, line_y=5.0, exit_state=True)
+    r = gpu_run(wk, wl, N, 3, spec=spec)
+    es = r["exit_state"]
+    p, v, w = es[0:3], es[3:6], es[6]
+    rd = v[2] < 0
+    assert w[rd].sum(dtype=np.float64) == pytest.approx(r["scalars"]["Rd"], rel=1e-6)
+    assert w[v[2] > 0].sum(dtype=np.float64) == pytest.approx(r["scalars"]["Tt"], rel=1e-6)
+    _, rdr = metrics.radialDistance(p[:, rd], w[rd], 50, 0.1, N)
+    da = np.array([2 * np.pi * (i + 0.5) * 0.1 ** 2 for i in range(50)])
+    assert np.allclose(rdr, r["rd_r"] / (da * N), rtol=1e-5, atol=1e-12)
+    _, rda = metrics.angularyResolved(v[:, rd], w[rd], 30, N)
+    d = np.pi / 60
+    alpha = np.array([(i + 0.5) * d for i in range(31)])
+    do = 4 * np.pi * np.sin(alpha) * np.sin(d / 2)
+    assert np.allclose(rda, r["rd_a"] / (do[:-1] * N), rtol=1e-4, atol=1e-9)
+    _, line = metrics.lineDistance(p[:, v[0] < 0], w[v[0] < 0], 40, 0.125, N, y_range=5.0)
+    assert np.allclose(line, r["line_z"] / (0.125 * 5.0 * 2 * N), rtol=1e-5, atol=1e-12)
+
+
+def test_known_answers_on_gpu(torch_cuda):
+    from pymonteopt_amd.voxel import VoxelPlateModel
+    ka = json.load(open(os.path.join(HERE, "golden", "reference_known_answers.json")))
+    c = ka["thin_slab"]
+    rd, tt = [], []
+    for s in range(10):
+        m = VoxelPlateModel(nPh=200000, nr=0, fluence=False)
+        m.build(**c["params"])
+        m.start(s)
+        v = m.getRdTtValues()
+        rd.append(v["Rd"])
+        tt.append(v["Td"])
+    se = lambda x: np.std(x, ddof=1) / math.sqrt(len(x))
+    assert abs(np.mean(rd) - c["literature"]["Rd"]) < 4 * se(rd)
+    assert abs(np.mean(tt) - c["literature"]["Tt"]) < 4 * se(tt)
+    c = ka["three_layer"]
+    m = VoxelPlateModel(nPh=c["n_packets"], nr=0, fluence=False)
+    m.build(**c["params"])
+    m.start(1)
+    v = m.getRdTtValues()
+    assert v["Rd"] == pytest.approx(c["reference_gpu"]["Rd"], abs=0.0015)
+    assert v["Td"] == pytest.approx(c["reference_gpu"]["Tt"], abs=0.0012)
+
+
+def test_skin_flat_beam_parity(torch_cuda, oracle):
+    """3-layer skin, 1 mm flat beam, 2e4 packets x 10 seeds against the oracle."""
+    from pymonteopt_amd import presets
+    from pymonteopt_amd.runtime import Walker
+    wl = presets.skin_plate("flat")
+    wk = Walker(wl.labels, wl.pitch, wl.optics)
+    N = 20000
+    G = [gpu_run(wk, wl, N, 200 + s) for s in range(10)]
+    O = [oracle_run(oracle, wl, N, s, absorbed=False) for s in range(10)]
+    for k in ("Rd", "Tt"):
+        assert zscore([g["scalars"][k] for g in G], [o["scalars"][k] for o in O]) < 3, k
+    zr = binned_z([g["rd_r"] for g in G], [o["rd_r"] for o in O])
+    assert np.sum(zr > 3) <= 2 and np.all(zr < 5)
+    for g in G:
+        assert ledger_error(g["scalars"]) < 1e-6
+
+
+def test_bone_phantom_parity(torch_cuda, oracle):
+    """64^3 bone phantom (GRF trabecular core, Fresnel at every bone/marrow face)."""
+    from pymonteopt_amd import geometry, presets
+    from pymonteopt_amd.presets import Workload
+    from pymonteopt_amd.runtime import Walker
+    vol = geometry.bone_phantom(seed=1, n=64, pitch=0.02, cortical=0.1, soft=0.2)
+    opt = geometry.bone_optics()
+    src = geometry.Source(voxel=(31, 31, 1), w0=float(1 - geometry.specular(1.0, 1.40)))
+    rings = tuple((0.1 * i, 0.1 * i + 0.1) for i in range(1, 6))
+    wl = Workload("bone 64^3", vol, 0.02, opt, src, dict(nr=30, dr=0.02, rings=rings))
+    wk = Walker(wl.labels, wl.pitch, wl.optics)
+    N = 20000
+    G = [gpu_run(wk, wl, N, 300 + s) for s in range(10)]
+    O = [oracle_run(oracle, wl, N, s) for s in range(10)]
+    for k in ("Rd", "Tt", "A"):
+        assert zscore([g["scalars"][k] for g in G], [o["scalars"][k] for o in O]) < 3, k
+    assert zscore([g["events"]["fresnel"] for g in G], [o["events"]["fresnel"] for o in O]) < 3
+    zr = binned_z([g["ring_w"] for g in G], [o["ring_w"] for o in O])
+    assert np.all(zr < 3.5)
+    for g in G:
+        assert ledger_error(g["scalars"]) < 1e-6
+
+
+def test_model_api_end_to_end(torch_cuda):
+    from pymonteopt_amd.voxel import VoxelPlateModel
+    m = VoxelPlateModel(nPh=20000, nr=50, dr=0.1, exit_state=True)
+    m.build(thickness=[5.0], ms=[10.0], ma=[0.1], g=[0.9], n=[1.37], n_air=1.0, x_size=10.1, y_size=10.1,
+            voxel_space=0.1)
+    res = m.start(7).get_result()
+    for k in ("Rd", "Tt", "A", "Rd_r", "Tt_r", "absorbed", "fluence", "p", "v", "w", "nPh", "energy"):
+        assert k in res, k
+    assert res["absorbed"].shape == (101, 101, 50)
+    e = res["energy"]
+    assert abs(e["w0"] - (e["Rd"] + e["Tt"] + e["other"] + e["A"] + e["kill"] - e["gain"])) < 1e-9
+    assert res["fluence"].max() > 0
