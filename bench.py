@@ -53,6 +53,7 @@ def main():
     ap.add_argument("--fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-absorbed", action="store_true")
+    ap.add_argument("--window", default="0,0,0", help="LDS tally window x,y,z (0 = default, -1 = none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
@@ -72,7 +73,8 @@ def main():
         spec.absorbed = False
     tal = wk.tallies(spec)
     N = int(args.packets)
-    kw = dict(block_size=args.block, packets_per_lane=args.ppl, fp64=args.fp64)
+    kw = dict(block_size=args.block, packets_per_lane=args.ppl, fp64=args.fp64,
+              window=tuple(int(x) for x in args.window.split(",")))
 
     for i in range(args.warmup):
         tal.zero()

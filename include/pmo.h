@@ -111,6 +111,9 @@ typedef struct pmo_run {
     int32_t block_size;     /* threads per block, multiple of 64, <= 1024; 0 = default         */
     int32_t grid_blocks;    /* persistent blocks; 0 = fill the device at full occupancy        */
     int32_t packets_per_lane; /* a wave claims 64*this packet ids per work-counter atomic; 0 = default */
+    int32_t window[3];      /* extents (x, y, z) of the box of the absorbed tally that each block
+                               accumulates in LDS, centred laterally on the launch voxel and
+                               starting at its z; {0,0,0} = default, {-1,-1,-1} = none          */
 } pmo_run;
 
 typedef struct pmo_tallies {

@@ -45,7 +45,8 @@ class PmoRun(C.Structure):
     _fields_ = [("n_packets", C.c_uint64), ("first_packet", C.c_uint64),
                 ("seed", C.c_uint32), ("stream", C.c_uint32),
                 ("wth", C.c_float), ("roulette_m", C.c_float), ("flags", C.c_uint32),
-                ("block_size", C.c_int32), ("grid_blocks", C.c_int32), ("packets_per_lane", C.c_int32)]
+                ("block_size", C.c_int32), ("grid_blocks", C.c_int32), ("packets_per_lane", C.c_int32),
+                ("window", C.c_int32 * 3)]
 
 
 class PmoTallies(C.Structure):

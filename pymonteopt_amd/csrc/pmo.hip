@@ -36,6 +36,7 @@ constexpr int kDefaultBlock = 256;
 constexpr int kDefaultPacketsPerLane = 4;
 constexpr uint32_t kMaxEvents = 1u << 28;
 constexpr size_t kMaxLds = 64 * 1024;
+constexpr int kDefaultWindow[3] = {16, 16, 8};
 
 struct DevInfo {
     int n_cu = 0;
@@ -113,6 +114,8 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
         return fail(PMO_ERR_INVALID, "volume extents (%d, %d, %d) must be >= 3", vol->nx, vol->ny, vol->nz);
     const unsigned long long nvox = (unsigned long long)vol->nx * vol->ny * vol->nz;
     if (nvox >= (1ull << 31)) return fail(PMO_ERR_UNSUPPORTED, "volume of %llu voxels exceeds 2^31", nvox);
+    if (vol->nx >= (1 << 24) || vol->ny >= (1 << 24) || vol->nz >= (1 << 24))
+        return fail(PMO_ERR_UNSUPPORTED, "volume extent >= 2^24");
     if (vol->n_labels < 1 || vol->n_labels > 255)
         return fail(PMO_ERR_LABEL, "n_labels = %d outside [1, 255]", vol->n_labels);
     if (!(vol->pitch > 0.f)) return fail(PMO_ERR_INVALID, "pitch must be > 0");
@@ -177,6 +180,30 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     P.exit_state = tal->exit_state;
     P.work = reinterpret_cast<unsigned long long*>(workspace);
 
+    // LDS window of the absorbed tally: the launch voxel region, where deposits collide
+    int wdim[3];
+    for (int a = 0; a < 3; ++a) {
+        wdim[a] = run->window[a] == 0 ? kDefaultWindow[a] : run->window[a];
+        if (wdim[a] < 0 || !P.absorbed) wdim[a] = 0;
+    }
+    if (wdim[0] > 0 && wdim[1] > 0 && wdim[2] > 0) {
+        const int ext[3] = {vol->nx, vol->ny, vol->nz};
+        const int ctr[3] = {src->ix, src->iy, src->iz};
+        int org[3];
+        for (int a = 0; a < 3; ++a) {
+            if (wdim[a] > ext[a]) wdim[a] = ext[a];
+            org[a] = a < 2 ? ctr[a] - wdim[a] / 2 : ctr[a];
+            if (org[a] < 0) org[a] = 0;
+            if (org[a] + wdim[a] > ext[a]) org[a] = ext[a] - wdim[a];
+        }
+        P.win_x0 = org[0]; P.win_y0 = org[1]; P.win_z0 = org[2];
+        P.win_nx = (uint32_t)wdim[0]; P.win_ny = (uint32_t)wdim[1]; P.win_nz = (uint32_t)wdim[2];
+    } else {
+        P.win_x0 = P.win_y0 = P.win_z0 = 0;
+        P.win_nx = P.win_ny = P.win_nz = 0;
+    }
+    P.win_cells = P.win_nx * P.win_ny * P.win_nz;
+
     // dynamic LDS: label table | block scalars | block events | histograms
     const size_t tab_bytes = (size_t)(vol->n_labels + 1) * (fp64 ? sizeof(pmo::LabelConst<double>)
                                                                   : sizeof(pmo::LabelConst<float>));
@@ -193,9 +220,10 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     P.h_line = off; off += P.line_z ? (uint32_t)P.nl : 0u;
     P.h_ring = off; off += P.ring_w ? (uint32_t)P.n_rings : 0u;
     P.h_total = off;
-    const size_t lds = P.lds_hist + (size_t)P.h_total * sizeof(double);
+    P.lds_win = (uint32_t)align_up(P.lds_hist + (size_t)P.h_total * sizeof(double), 16);
+    const size_t lds = P.lds_win + (size_t)P.win_cells * sizeof(double);
     if (lds > kMaxLds)
-        return fail(PMO_ERR_UNSUPPORTED, "exit histograms need %zu B of LDS (limit %zu); use fewer bins", lds, kMaxLds);
+        return fail(PMO_ERR_UNSUPPORTED, "histograms and tally window need %zu B of LDS (limit %zu)", lds, kMaxLds);
 
     int nb = 0;
     rc = fp64 ? blocks_per_cu<double>(bs, lds, &nb) : blocks_per_cu<float>(bs, lds, &nb);

@@ -107,8 +107,11 @@ struct WalkParams {
     double* ring_w;
     float* exit_state;
     unsigned long long* work;
+    // LDS-privatised box of the absorbed tally around the launch voxel
+    int32_t win_x0, win_y0, win_z0;
+    uint32_t win_nx, win_ny, win_nz, win_cells;
     // dynamic LDS layout (bytes)
-    uint32_t lds_tab, lds_scal, lds_ev, lds_hist;
+    uint32_t lds_tab, lds_scal, lds_ev, lds_hist, lds_win;
     uint32_t h_rd, h_tt, h_top, h_rda, h_tta, h_line, h_ring, h_total;  // offsets in doubles
 };
 
@@ -122,6 +125,17 @@ __device__ __forceinline__ void lds_add(unsigned long long* p, unsigned long lon
 __device__ __forceinline__ void gmem_add(double* p, double v)
 {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// a pending absorbed-weight deposit lives in one of two places: slot >= 0 is a voxel index
+// of the global tally, slot < 0 is -(cell + 1) of the block's LDS window
+constexpr int32_t kNoSlot = 0x7fffffff;
+
+__device__ __forceinline__ void flush_deposit(const WalkParams& P, double* win, int32_t slot, double v)
+{
+    if (slot == kNoSlot) return;
+    if (slot < 0) lds_add(win + (-slot - 1), v);
+    else gmem_add(P.absorbed + slot, v);
 }
 
 __device__ __forceinline__ int bin_of(float x, float width, int nbins)
@@ -142,6 +156,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(const WalkParams P)
     double* bscal = reinterpret_cast<double*>(smem + P.lds_scal);
     unsigned long long* bev = reinterpret_cast<unsigned long long*>(smem + P.lds_ev);
     double* hist = reinterpret_cast<double*>(smem + P.lds_hist);
+    double* win = reinterpret_cast<double*>(smem + P.lds_win);
 
     const uint32_t end = P.n_labels;
     for (uint32_t i = threadIdx.x; i <= end; i += blockDim.x) {
@@ -163,6 +178,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(const WalkParams P)
     for (uint32_t i = threadIdx.x; i < kBS_COUNT; i += blockDim.x) bscal[i] = 0.0;
     for (uint32_t i = threadIdx.x; i < kBE_COUNT; i += blockDim.x) bev[i] = 0ull;
     for (uint32_t i = threadIdx.x; i < P.h_total; i += blockDim.x) hist[i] = 0.0;
+    for (uint32_t i = threadIdx.x; i < P.win_cells; i += blockDim.x) win[i] = 0.0;
     __syncthreads();
 
     const uint32_t lane = threadIdx.x & 63u;
@@ -176,11 +192,12 @@ __global__ void __launch_bounds__(1024) walk_kernel(const WalkParams P)
     R px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0, ivx = 0, ivy = 0, ivz = 0;
     R w = 0, st = 0, mt = 0, ni = 0, u_roul = 0, u_th = 0, u_phi = 0;
     R fb0 = 0, fb1 = 0, fb2 = 0;
-    uint32_t vi = 0, lab = 0, inter = 0, fres = 0, nev = 0;
+    int32_t ax = 0, ay = 0, az = 0;    // voxel address (reference add[0..2])
+    uint32_t lab = 0, inter = 0, fres = 0, nev = 0;
     unsigned long long pid = 0;
     // lane accumulators (kept across packets)
     double acc_abs = 0.0;
-    uint32_t dep_vi = 0xffffffffu;
+    int32_t dep_slot = kNoSlot;      // where the pending deposit goes (see flush_deposit)
     double dep_acc = 0.0;
     uint32_t n_cross = 0, n_pk = 0;
 
@@ -210,7 +227,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(const WalkParams P)
                     // ---------------- launch a packet (voxel/_classes.py:107-213)
                     pid = P.first_packet + b_next + rank;
                     const uint32_t plo = (uint32_t)pid, phi = (uint32_t)(pid >> 32);
-                    int32_t ax = P.ix, ay = P.iy;
+                    ax = P.ix; ay = P.iy; az = P.iz;
                     px = 0; py = 0; pz = -half;
                     if (P.src_kind != 0) {
                         const uint4 r = philox4x32_10(make_uint4(0u, kTagSource, plo, phi), P.seed, P.stream);
@@ -232,7 +249,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(const WalkParams P)
                         if (ay < 1) { ay = 1; py = -half; }
                         if (ay > (int)P.ny - 2) { ay = (int)P.ny - 2; py = half; }
                     }
-                    vi = ((uint32_t)ax * P.ny + (uint32_t)ay) * P.nz + (uint32_t)P.iz;
+                    const uint32_t vi = ((uint32_t)ax * P.ny + (uint32_t)ay) * P.nz + (uint32_t)az;
                     vx = (R)P.dirx; vy = (R)P.diry; vz = (R)P.dirz;
                     ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
                     w = (R)P.w0;
@@ -281,10 +298,15 @@ __global__ void __launch_bounds__(1024) walk_kernel(const WalkParams P)
                 if (k == 0) px = face; else if (k == 1) py = face; else pz = face;
                 st -= smt;
                 const int s = vk > (R)0 ? 1 : -1;
-                const uint32_t stride = k == 0 ? P.sx : (k == 1 ? P.sy : 1u);
-                const uint32_t nvi = s > 0 ? vi + stride : vi - stride;
-                uint32_t nxt = nvi < P.nvox ? (uint32_t)P.labels[nvi] : end;
-                nxt = nxt < end ? nxt : end;
+                const int32_t bx = ax + (k == 0 ? s : 0);
+                const int32_t by = ay + (k == 1 ? s : 0);
+                const int32_t bz = az + (k == 2 ? s : 0);
+                uint32_t nxt = end;
+                if ((uint32_t)bx < P.nx && (uint32_t)by < P.ny && (uint32_t)bz < P.nz) {
+                    const uint32_t nvi = (__umul24((uint32_t)bx, P.ny) + (uint32_t)by) * P.nz + (uint32_t)bz;
+                    nxt = P.labels[nvi];
+                    nxt = nxt < end ? nxt : end;
+                }
                 const R nt = tab[nxt].n;
                 bool transmit = true;
                 if (ni != nt) {
@@ -334,7 +356,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(const WalkParams P)
                     }
                 }
                 if (transmit) {
-                    vi = nvi;
+                    ax = bx; ay = by; az = bz;
                     if (k == 0) px = -px; else if (k == 1) py = -py; else pz = -pz;
                     lab = nxt;
                     if (lab == end) {
@@ -356,11 +378,18 @@ __global__ void __launch_bounds__(1024) walk_kernel(const WalkParams P)
                 w = wn;
                 acc_abs += dep;
                 if (P.absorbed) {
-                    if (vi == dep_vi) {
+                    const uint32_t lx = (uint32_t)(ax - P.win_x0), ly = (uint32_t)(ay - P.win_y0),
+                                   lz = (uint32_t)(az - P.win_z0);
+                    int32_t slot;
+                    if (lx < P.win_nx && ly < P.win_ny && lz < P.win_nz)
+                        slot = -(int32_t)((lx * P.win_ny + ly) * P.win_nz + lz) - 1;
+                    else
+                        slot = (int32_t)((__umul24((uint32_t)ax, P.ny) + (uint32_t)ay) * P.nz + (uint32_t)az);
+                    if (slot == dep_slot) {
                         dep_acc += dep;
                     } else {
-                        if (dep_vi != 0xffffffffu) gmem_add(P.absorbed + dep_vi, dep_acc);
-                        dep_vi = vi;
+                        flush_deposit(P, win, dep_slot, dep_acc);
+                        dep_slot = slot;
                         dep_acc = dep;
                     }
                 }
@@ -419,10 +448,6 @@ __global__ void __launch_bounds__(1024) walk_kernel(const WalkParams P)
             lds_add(&bev[kBE_INTERACT], (unsigned long long)inter);
             lds_add(&bev[kBE_FRESNEL], (unsigned long long)fres);
             const uint64_t slot = pid - P.first_packet;
-            const uint32_t az = vi % P.nz;
-            const uint32_t t2 = vi / P.nz;
-            const uint32_t ay = t2 % P.ny;
-            const uint32_t ax = t2 / P.ny;
             if (killed) {
                 lds_add(&bev[kBE_KILLED], 1ull);
                 lds_add(&bscal[kBS_KILL], (double)w);
@@ -439,9 +464,9 @@ __global__ void __launch_bounds__(1024) walk_kernel(const WalkParams P)
                 lds_add(&bev[kBE_ESCAPED], 1ull);
                 const double wd = (double)w;
                 // exit encoding, voxel/_classes.py:261-269
-                const float x = P.pitch * (float)((int32_t)ax - P.cx) + (float)px;
-                const float y = P.pitch * (float)((int32_t)ay - P.cy) + (float)py;
-                const float z = P.pitch * (float)((int32_t)az - 1) + (float)pz + 0.5f * P.pitch;
+                const float x = P.pitch * (float)(ax - P.cx) + (float)px;
+                const float y = P.pitch * (float)(ay - P.cy) + (float)py;
+                const float z = P.pitch * (float)(az - 1) + (float)pz + 0.5f * P.pitch;
                 const float fvz = (float)vz;
                 lds_add(&bscal[fvz < 0.f ? kBS_RD : (fvz > 0.f ? kBS_TT : kBS_OTHER)], wd);
                 lds_add(&bscal[kBS_FACE + 2 * xk + (xs > 0 ? 1 : 0)], wd);
@@ -484,7 +509,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(const WalkParams P)
     }
 
     // ---------------------------------------------------------------- block flush
-    if (P.absorbed && dep_vi != 0xffffffffu) gmem_add(P.absorbed + dep_vi, dep_acc);
+    if (P.absorbed) flush_deposit(P, win, dep_slot, dep_acc);
     lds_add(&bscal[kBS_ABS], acc_abs);
     lds_add(&bscal[kBS_W0], (double)n_pk * (double)P.w0);
     lds_add(&bev[kBE_CROSS], (unsigned long long)n_cross);
@@ -507,6 +532,14 @@ __global__ void __launch_bounds__(1024) walk_kernel(const WalkParams P)
         else if (i < P.h_ring) dst = P.line_z + (i - P.h_line);
         else dst = P.ring_w + (i - P.h_ring);
         gmem_add(dst, v);
+    }
+    for (uint32_t i = threadIdx.x; i < P.win_cells; i += blockDim.x) {
+        const double v = win[i];
+        if (v == 0.0) continue;
+        const uint32_t lz = i % P.win_nz, t = i / P.win_nz;
+        const uint32_t ly = t % P.win_ny, lx = t / P.win_ny;
+        const uint32_t g = (((uint32_t)P.win_x0 + lx) * P.ny + (uint32_t)P.win_y0 + ly) * P.nz + (uint32_t)P.win_z0 + lz;
+        gmem_add(P.absorbed + g, v);
     }
 }
 

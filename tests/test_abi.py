@@ -39,7 +39,7 @@ PROBE = r"""
 int main(void) {
   S(pmo_volume) S(pmo_source) S(pmo_run) S(pmo_tallies)
   O(pmo_volume, optics) O(pmo_source, w0) O(pmo_run, packets_per_lane) O(pmo_tallies, exit_state)
-  O(pmo_tallies, ring_w) O(pmo_tallies, line_z) O(pmo_run, flags)
+  O(pmo_tallies, ring_w) O(pmo_tallies, line_z) O(pmo_run, flags) O(pmo_run, window)
   printf("PMO_S_COUNT %d\nPMO_EV_COUNT %d\n", PMO_S_COUNT, PMO_EV_COUNT);
   return 0;
 }
@@ -60,6 +60,7 @@ def test_ctypes_layout_matches_header(tmp_path):
     assert int(out["pmo_source.w0"]) == _abi.PmoSource.w0.offset
     assert int(out["pmo_run.packets_per_lane"]) == _abi.PmoRun.packets_per_lane.offset
     assert int(out["pmo_run.flags"]) == _abi.PmoRun.flags.offset
+    assert int(out["pmo_run.window"]) == _abi.PmoRun.window.offset
     assert int(out["pmo_tallies.exit_state"]) == _abi.PmoTallies.exit_state.offset
     assert int(out["pmo_tallies.ring_w"]) == _abi.PmoTallies.ring_w.offset
     assert int(out["pmo_tallies.line_z"]) == _abi.PmoTallies.line_z.offset
