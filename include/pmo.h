@@ -114,6 +114,8 @@ typedef struct pmo_run {
     int32_t window[3];      /* extents (x, y, z) of the box of the absorbed tally that each block
                                accumulates in LDS, centred laterally on the launch voxel and
                                starting at its z; {0,0,0} = default, {-1,-1,-1} = none          */
+    const uint32_t* key_dev; /* device (seed, stream) read by the kernel instead of seed/stream when
+                                non-NULL, so a captured HIP graph can be replayed with new keys */
 } pmo_run;
 
 typedef struct pmo_tallies {
@@ -139,6 +141,30 @@ typedef struct pmo_tallies {
                                get_result encoding, indexed by packet id - first_packet, or NULL */
 } pmo_tallies;
 
+/* ---- batched plates ----------------------------------------------------------------- */
+/* One instance of a batched sweep: a plate (pymopt/voxel/_classes.py:352-424 PlateModel
+ * geometry) of nx x ny x nz padded voxels whose interior label depends on z only. */
+typedef struct pmo_plate_instance {
+    const uint8_t* z_labels; /* device [nz]: label of each z slice; slices 0 and nz-1 hold the end
+                                label, as does the lateral 1-voxel shell                          */
+    const float* optics;     /* device float4 [n_labels+1] (mu_a, mu_s, g, n), as pmo_volume      */
+    int32_t nx, ny, nz;
+    int32_t n_labels;
+    float pitch;
+    int32_t src_kind, ix, iy, iz; /* launch (pmo_source fields)                                 */
+    float radius, dir[3], w0;
+    uint64_t n_packets;      /* this instance's budget (< 2^32)                                    */
+} pmo_plate_instance;
+
+typedef struct pmo_batch_tallies {
+    double* scalars;         /* device [n_inst][PMO_S_COUNT], required                            */
+    uint64_t* events;        /* device [PMO_EV_COUNT] summed over instances, or NULL             */
+    int32_t nr;              /* radial bins per instance (i*dr, (i+1)*dr]                         */
+    float dr;
+    double* rd_r;            /* device [n_inst][nr] v_z < 0 exits, or NULL                       */
+    double* tt_r;            /* device [n_inst][nr] v_z > 0 exits, or NULL                       */
+} pmo_batch_tallies;
+
 /* ---- library ------------------------------------------------------------------------ */
 int pmo_abi_version(void);
 const char* pmo_last_error(void);
@@ -160,6 +186,24 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run,
  * 4*n uint32. */
 int pmo_philox(const uint32_t* ctr, uint32_t key0, uint32_t key1, uint32_t* out, int64_t n,
                void* stream);
+
+/* Host helper: the round-robin batch schedule of a sweep. Writes up to `cap` batches
+ * (instance, first local packet id, count, 0) into `sched` (host, 4*cap uint32), cycling over
+ * the instances with at most `batch` packets per entry. Returns the number of batches the
+ * sweep needs (call with cap = 0 to size the buffer) or a negative error code. */
+int64_t pmo_plate_schedule(const uint64_t* n_packets, int32_t n_inst, int32_t batch, uint32_t* sched,
+                           int64_t cap);
+
+/* Many plates in one persistent launch. Packet j of instance i draws from Philox key
+ * (seed, stream + i) with packet id j, i.e. exactly the draws of a plain pmo_walk of that
+ * plate with stream + i, so the sweep reproduces the sequential runs it replaces: one
+ * VoxelPlateModel build/start per parameter set, as the reference's optical-property sweep
+ * does (.ipynb_checkpoints/si_in_光学特性推定-checkpoint.ipynb:13945-13967).
+ * `inst` is a device array of n_inst descriptors, `schedule` the device copy of
+ * pmo_plate_schedule's output. run->n_packets, first_packet and window are ignored. */
+int pmo_walk_plates(const pmo_plate_instance* inst, int32_t n_inst, const uint32_t* schedule,
+                    int64_t n_batches, const pmo_run* run, const pmo_batch_tallies* tal, void* workspace,
+                    void* stream);
 
 #ifdef __cplusplus
 }

@@ -46,7 +46,7 @@ class PmoRun(C.Structure):
                 ("seed", C.c_uint32), ("stream", C.c_uint32),
                 ("wth", C.c_float), ("roulette_m", C.c_float), ("flags", C.c_uint32),
                 ("block_size", C.c_int32), ("grid_blocks", C.c_int32), ("packets_per_lane", C.c_int32),
-                ("window", C.c_int32 * 3)]
+                ("window", C.c_int32 * 3), ("key_dev", C.c_void_p)]
 
 
 class PmoTallies(C.Structure):
@@ -59,6 +59,18 @@ class PmoTallies(C.Structure):
                 ("exit_state", C.c_void_p)]
 
 
+class PmoPlateInstance(C.Structure):
+    _fields_ = [("z_labels", C.c_void_p), ("optics", C.c_void_p), ("nx", C.c_int32), ("ny", C.c_int32),
+                ("nz", C.c_int32), ("n_labels", C.c_int32), ("pitch", C.c_float), ("src_kind", C.c_int32),
+                ("ix", C.c_int32), ("iy", C.c_int32), ("iz", C.c_int32), ("radius", C.c_float),
+                ("dir", C.c_float * 3), ("w0", C.c_float), ("n_packets", C.c_uint64)]
+
+
+class PmoBatchTallies(C.Structure):
+    _fields_ = [("scalars", C.c_void_p), ("events", C.c_void_p), ("nr", C.c_int32), ("dr", C.c_float),
+                ("rd_r", C.c_void_p), ("tt_r", C.c_void_p)]
+
+
 # every symbol include/pmo.h declares, with its ctypes signature
 SIGNATURES = {
     "pmo_abi_version": ([], C.c_int),
@@ -68,6 +80,9 @@ SIGNATURES = {
     "pmo_walk": ([C.POINTER(PmoVolume), C.POINTER(PmoSource), C.POINTER(PmoRun), C.POINTER(PmoTallies),
                   C.c_void_p, C.c_void_p], C.c_int),
     "pmo_philox": ([C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
+    "pmo_plate_schedule": ([C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int64], C.c_int64),
+    "pmo_walk_plates": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.POINTER(PmoRun),
+                         C.POINTER(PmoBatchTallies), C.c_void_p, C.c_void_p], C.c_int),
 }
 
 _lib = None

@@ -1,0 +1,111 @@
+"""Launch-shape autotuner for the walk kernels.
+
+Times short runs of a workload with HIP events over a grid of block sizes and
+packets-per-lane (plain walk) or schedule batch sizes (batched sweep) and keeps the
+fastest. Results are cached per (device, workload signature) for the life of the process
+and, when a path is given, in a JSON file.
+"""
+from __future__ import annotations
+
+import json
+import os
+
+import torch
+
+_CACHE: dict = {}
+
+
+def _time(fn, reps=2):
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fn()                                   # warm-up (code-object load, caches)
+    best = float("inf")
+    for _ in range(reps):
+        e0.record()
+        fn()
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e-3)
+    return best
+
+
+def _load(path):
+    if path and os.path.exists(path):
+        try:
+            return json.load(open(path))
+        except (OSError, ValueError):
+            return {}
+    return {}
+
+
+def _store(path, key, value):
+    if not path:
+        return
+    d = _load(path)
+    d[key] = value
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(d, f, indent=1)
+    os.replace(tmp, path)
+
+
+def tune_walk(walker, source, spec, n_packets, *, sample=2_000_000, blocks=(128, 256, 512, 1024),
+              ppls=(1, 2, 4, 8), seed=0, cache_path=None, **walk_kw):
+    """Best (block_size, packets_per_lane) for `walker.walk` on this workload.
+
+    The sample keeps the workload's packets-per-resident-wave regime: runs of n_packets
+    are timed when they are small, else `sample` packets."""
+    dev = torch.cuda.get_device_name(walker.device)
+    key = f"walk|{dev}|{walker.shape}|{walker.pitch}|{walker.optics.cpu().numpy().tobytes().hex()[:64]}|" \
+          f"{source.kind}|{int(n_packets)}|{spec.absorbed}"
+    if key in _CACHE:
+        return _CACHE[key]
+    disk = _load(cache_path)
+    if key in disk:
+        _CACHE[key] = disk[key]
+        return disk[key]
+    n = int(min(n_packets, sample))
+    tal = walker.tallies(spec)
+    table = []
+    for bs in blocks:
+        for ppl in ppls:
+            def run():
+                tal.zero()
+                walker.walk(source, n, seed, tal, block_size=bs, packets_per_lane=ppl, **walk_kw)
+            t = _time(run)
+            table.append({"block_size": bs, "packets_per_lane": ppl, "seconds": t, "photons_per_s": n / t})
+    best = min(table, key=lambda r: r["seconds"])
+    res = {"block_size": best["block_size"], "packets_per_lane": best["packets_per_lane"],
+           "photons_per_s": best["photons_per_s"], "sample_packets": n, "table": table}
+    _CACHE[key] = res
+    _store(cache_path, key, res)
+    return res
+
+
+def tune_sweep(sweep, *, blocks=(128, 256, 512, 1024), batches=(64, 128, 256, 512, 1024), seed=0,
+               cache_path=None):
+    """Best (block_size, batch) for a PlateSweep; leaves the sweep set to the winner."""
+    dev = torch.cuda.get_device_name(sweep.device)
+    sig = tuple((tuple(p.thickness), tuple(p.ma), tuple(p.ms), p.n_packets) for p in sweep.plates[:8])
+    key = f"sweep|{dev}|{sweep.n_inst}|{sweep.total_packets}|{hash(sig)}"
+    if key in _CACHE:
+        res = _CACHE[key]
+    else:
+        res = _load(cache_path).get(key)
+    if res is None:
+        table = []
+        orig = sweep.batch
+        for b in batches:
+            sweep.set_batch(b)
+            for bs in blocks:
+                t = _time(lambda: sweep.run(seed, block_size=bs), reps=1)
+                table.append({"block_size": bs, "batch": b, "seconds": t,
+                              "photons_per_s": sweep.total_packets / t})
+        sweep.set_batch(orig)
+        best = min(table, key=lambda r: r["seconds"])
+        res = {"block_size": best["block_size"], "batch": best["batch"], "photons_per_s": best["photons_per_s"],
+               "table": table}
+        _CACHE[key] = res
+        _store(cache_path, key, res)
+    sweep.set_batch(res["batch"])
+    sweep.block_size = res["block_size"]
+    return res

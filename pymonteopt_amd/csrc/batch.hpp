@@ -1,0 +1,294 @@
+// Batched walk over many layered plates in one persistent launch (gfx950).
+//
+// Each instance is a plate (pymopt/voxel/_classes.py:352-424 PlateModel geometry): a box of
+// nx x ny x nz padded voxels whose interior label depends on z only, given as a column of
+// nz labels (slices 0 and nz-1 and the lateral shell are the end label). A homogeneous slab
+// is a one-layer plate. Instances have their own optics, launch source and packet budget.
+//
+// Work is a schedule of packet batches (instance, first local id, count), ordered
+// round-robin over instances so that every instance's tallies are updated throughout the
+// launch instead of in one burst. A wave claims one batch per counter atomic. Packet
+// (instance i, local id j) draws from Philox key (seed, stream + i) with packet id j, the
+// same draws as a plain walk of instance i alone with stream + i, so each instance's
+// result is that of its own plain walk.
+#pragma once
+
+#include "walk.hpp"
+
+namespace pmo {
+
+struct PlateInstance {     // device layout of pmo_plate_instance (include/pmo.h)
+    const uint8_t* z_labels;
+    const float4* optics;
+    int32_t nx, ny, nz, n_labels;
+    float pitch;
+    int32_t src_kind, ix, iy, iz;
+    float radius, dirx, diry, dirz, w0;
+    unsigned long long n_packets;
+};
+
+struct BatchParams {
+    const PlateInstance* inst;
+    int32_t n_inst;
+    const uint4* schedule;      // (instance, first local id, count, 0)
+    unsigned long long n_batches;
+    uint32_t seed, stream;
+    float wth, roulette_m, inv_m;
+    uint32_t flags;
+    uint32_t max_events;
+    double* scalars;            // [n_inst][kBS_COUNT]
+    unsigned long long* events; // [kBE_COUNT]
+    int32_t nr;
+    float dr;
+    double* rd_r;               // [n_inst][nr]
+    double* tt_r;               // [n_inst][nr]
+    unsigned long long* work;
+    const uint32_t* key_dev;
+};
+
+// the fields of the current instance a lane needs on every event, kept in registers
+struct PlateGeo {
+    const uint8_t* z_labels;
+    const float4* optics;
+    int32_t nx, ny, nz;
+    uint32_t end;
+};
+
+__device__ __forceinline__ uint32_t plate_label(const PlateGeo& G, int32_t bx, int32_t by, int32_t bz)
+{
+    if (bx < 1 || by < 1 || bx > G.nx - 2 || by > G.ny - 2 || (uint32_t)bz >= (uint32_t)G.nz) return G.end;
+    const uint32_t l = G.z_labels[bz];
+    return l < G.end ? l : G.end;
+}
+
+// label across the face on axis k: in a plate only z-crossings can change layers, so
+// lateral crossings keep the label unless they enter the lateral shell
+__device__ __forceinline__ uint32_t plate_next_label(const PlateGeo& G, uint32_t lab, int k, int32_t bx, int32_t by,
+                                                     int32_t bz)
+{
+    if (k == 2) return plate_label(G, bx, by, bz);
+    return (bx < 1 || by < 1 || bx > G.nx - 2 || by > G.ny - 2) ? G.end : lab;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5)))
+plate_batch_kernel(const BatchParams P)
+{
+    using M = Mth<R>;
+    __shared__ unsigned long long bev[kBE_COUNT];
+    for (uint32_t i = threadIdx.x; i < kBE_COUNT; i += blockDim.x) bev[i] = 0ull;
+    __syncthreads();
+
+    const uint32_t lane = threadIdx.x & 63u;
+    const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+    const bool normal_limit = (P.flags & 2u) != 0u;
+    const uint32_t seed = P.key_dev ? __builtin_amdgcn_readfirstlane(P.key_dev[0]) : P.seed;
+    const uint32_t skey = P.key_dev ? __builtin_amdgcn_readfirstlane(P.key_dev[1]) : P.stream;
+
+    bool alive = false;
+    R px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0, ivx = 0, ivy = 0, ivz = 0;
+    R w = 0, st = 0, u_roul = 0, u_th = 0, u_phi = 0, fb0 = 0, fb1 = 0, fb2 = 0, half = 0;
+    LabelConst<R> L;                  // optics of the current label
+    PlateGeo G{nullptr, nullptr, 0, 0, 0, 0u};
+    int32_t ax = 0, ay = 0, az = 0;
+    uint32_t lab = 0, inter = 0, fres = 0, nev = 0, pid = 0, key1 = 0;
+    int32_t inst = -1;
+    double acc_abs = 0.0, acc_w0 = 0.0;
+    uint32_t n_cross = 0;
+
+    // wave-uniform current batch
+    uint32_t b_inst = 0, b_next = 0, b_end = 0;
+    bool exhausted = false;
+
+    for (;;) {
+        const unsigned long long need = __ballot(!alive);
+        if (need != 0ull && !exhausted) {
+            if (b_next == b_end) {
+                unsigned long long claim = 0;
+                if (lane == 0) claim = atomicAdd(P.work, 1ull);
+                claim = __shfl(claim, 0);
+                if (claim >= P.n_batches) {
+                    exhausted = true;
+                } else {
+                    const uint4 b = P.schedule[claim];
+                    b_inst = b.x < (uint32_t)P.n_inst ? b.x : 0u;
+                    b_next = b.y;
+                    b_end = b.x < (uint32_t)P.n_inst ? b.y + b.z : b.y;
+                }
+            }
+            if (!exhausted) {
+                const uint32_t avail = b_end - b_next;
+                const uint32_t rank = (uint32_t)__popcll(need & lt_mask);
+                if (!alive && rank < avail) {
+                    if ((int32_t)b_inst != inst) {
+                        // leaving an instance: flush its per-lane sums
+                        if (inst >= 0) {
+                            gmem_add(P.scalars + (size_t)inst * kBS_COUNT + kBS_ABS, acc_abs);
+                            gmem_add(P.scalars + (size_t)inst * kBS_COUNT + kBS_W0, acc_w0);
+                        }
+                        acc_abs = 0.0; acc_w0 = 0.0;
+                        inst = (int32_t)b_inst;
+                    }
+                    const PlateInstance& I = P.inst[inst];
+                    G.z_labels = I.z_labels; G.optics = I.optics;
+                    G.nx = I.nx; G.ny = I.ny; G.nz = I.nz; G.end = (uint32_t)I.n_labels;
+                    pid = b_next + rank;
+                    key1 = skey + (uint32_t)inst;
+                    half = (R)I.pitch * (R)0.5;
+                    ax = I.ix; ay = I.iy; az = I.iz;
+                    px = 0; py = 0; pz = -half;
+                    if (I.src_kind != 0)
+                        launch_xy<R>(I.src_kind, I.radius, (R)I.pitch, half, pid, 0u, seed, key1, I.nx, I.ny, ax,
+                                     ay, px, py);
+                    vx = (R)I.dirx; vy = (R)I.diry; vz = (R)I.dirz;
+                    ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
+                    w = (R)I.w0;
+                    acc_w0 += (double)I.w0;
+                    lab = plate_label(G, ax, ay, az);
+                    L = label_const<R>(G.optics[lab]);
+                    inter = 0; fres = 0; nev = 0;
+                    const uint4 r = philox4x32_10(make_uint4(0u, kTagWalk, pid, 0u), seed, key1);
+                    st = M::neglog(u01<R>(r.x));
+                    u_roul = u01<R>(r.y); u_th = u01<R>(r.z); u_phi = u01<R>(r.w);
+                    alive = true;
+                }
+                b_next += (uint32_t)__popcll(need) < avail ? (uint32_t)__popcll(need) : avail;
+            }
+        }
+        if (__ballot(alive) == 0ull) break;
+        if (!alive) continue;
+
+        const uint32_t end = G.end;
+        bool ended = false, killed = false;
+        int xk = 2, xs = 1;
+        if (lab == end) {
+            ended = true; xs = vz > (R)0 ? 1 : -1;
+        } else {
+            ++nev;
+            const R d0 = (half - (vx > (R)0 ? px : -px)) * ivx;
+            const R d1 = (half - (vy > (R)0 ? py : -py)) * ivy;
+            const R d2 = (half - (vz > (R)0 ? pz : -pz)) * ivz;
+            R db = (R)1000;
+            int k = 0;
+            if (d0 < db) { db = d0; k = 0; }
+            if (d1 < db) { db = d1; k = 1; }
+            if (d2 < db) { db = d2; k = 2; }
+            const R smt = db * L.mt;
+            if (st >= smt) {
+                ++n_cross;
+                px += vx * db; py += vy * db; pz += vz * db;
+                const R vk = k == 0 ? vx : (k == 1 ? vy : vz);
+                const R face = M::copysign(half, vk);
+                if (k == 0) px = face; else if (k == 1) py = face; else pz = face;
+                st -= smt;
+                const int s = vk > (R)0 ? 1 : -1;
+                const int32_t bx = ax + (k == 0 ? s : 0);
+                const int32_t by = ay + (k == 1 ? s : 0);
+                const int32_t bz = az + (k == 2 ? s : 0);
+                const uint32_t nxt = plate_next_label(G, lab, k, bx, by, bz);
+                bool transmit = true;
+                LabelConst<R> Ln = L;
+                if (nxt != lab) {
+                    Ln = label_const<R>(G.optics[nxt]);
+                    if (L.n != Ln.n) {
+                        R u;
+                        if ((fres & 3u) == 0u) {
+                            const uint4 r = philox4x32_10(make_uint4(fres >> 2, kTagFresnel, pid, 0u), seed, key1);
+                            u = u01<R>(r.x); fb0 = u01<R>(r.y); fb1 = u01<R>(r.z); fb2 = u01<R>(r.w);
+                        } else {
+                            u = fb0; fb0 = fb1; fb1 = fb2;
+                        }
+                        ++fres;
+                        R ct;
+                        const R rf = fresnel_reflectance<R>(L.n, Ln.n, M::fabs(vk), normal_limit, ct);
+                        if (rf < u) {
+                            refract<R>(k, vk, L.n / Ln.n, ct, vx, vy, vz);
+                            ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
+                        } else {
+                            transmit = false;
+                        }
+                    }
+                }
+                if (transmit) {
+                    ax = bx; ay = by; az = bz;
+                    if (k == 0) px = -px; else if (k == 1) py = -py; else pz = -pz;
+                    if (nxt == end) {
+                        ended = true; xk = k; xs = s;
+                    } else if (nxt != lab) {
+                        L = Ln;
+                    }
+                    lab = nxt;
+                } else {
+                    if (k == 0) vx = -vx; else if (k == 1) vy = -vy; else vz = -vz;
+                }
+            } else {
+                const R sm = M::div(st, L.mt);
+                px += vx * sm; py += vy * sm; pz += vz * sm;
+                const R wn = w - w * L.ra;
+                acc_abs += (double)w - (double)wn;
+                w = wn;
+                if (w <= (R)P.wth) {
+                    atomicAdd(&bev[kBE_ROULETTE], 1ull);
+                    if ((R)P.inv_m < u_roul) {
+                        killed = true;
+                    } else {
+                        const R wm = w * (R)P.roulette_m;
+                        gmem_add(P.scalars + (size_t)inst * kBS_COUNT + kBS_GAIN, (double)wm - (double)w);
+                        w = wm;
+                    }
+                }
+                if (!killed) {
+                    spin<R>(hg_cos<R>(L, u_th), u_phi, vx, vy, vz);
+                    ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
+                    ++inter;
+                    const uint4 r = philox4x32_10(make_uint4(inter, kTagWalk, pid, 0u), seed, key1);
+                    st = M::neglog(u01<R>(r.x));
+                    u_roul = u01<R>(r.y); u_th = u01<R>(r.z); u_phi = u01<R>(r.w);
+                } else {
+                    ++inter;
+                }
+            }
+            if (!ended && !killed && nev >= P.max_events) killed = true;
+        }
+
+        if (ended || killed) {
+            atomicAdd(&bev[kBE_INTERACT], (unsigned long long)inter);
+            atomicAdd(&bev[kBE_FRESNEL], (unsigned long long)fres);
+            atomicAdd(&bev[kBE_PACKETS], 1ull);
+            double* sc = P.scalars + (size_t)inst * kBS_COUNT;
+            if (killed) {
+                atomicAdd(&bev[kBE_KILLED], 1ull);
+                gmem_add(sc + kBS_KILL, (double)w);
+            } else {
+                atomicAdd(&bev[kBE_ESCAPED], 1ull);
+                const double wd = (double)w;
+                const float fvz = (float)vz;
+                gmem_add(sc + (fvz < 0.f ? kBS_RD : (fvz > 0.f ? kBS_TT : kBS_OTHER)), wd);
+                gmem_add(sc + kBS_FACE + 2 * xk + (xs > 0 ? 1 : 0), wd);
+                if (P.nr > 0 && fvz != 0.f) {
+                    const float pitch = P.inst[inst].pitch;
+                    const int cx = (G.nx - 1) / 2, cy = (G.ny - 1) / 2;
+                    const float x = pitch * (float)(ax - cx) + (float)px;
+                    const float y = pitch * (float)(ay - cy) + (float)py;
+                    const int b = bin_of(__builtin_sqrtf(x * x + y * y), P.dr, P.nr);
+                    if (b >= 0) {
+                        double* h = fvz < 0.f ? P.rd_r : P.tt_r;
+                        if (h) gmem_add(h + (size_t)inst * P.nr + b, wd);
+                    }
+                }
+            }
+            alive = false;
+        }
+    }
+    if (inst >= 0) {
+        gmem_add(P.scalars + (size_t)inst * kBS_COUNT + kBS_ABS, acc_abs);
+        gmem_add(P.scalars + (size_t)inst * kBS_COUNT + kBS_W0, acc_w0);
+    }
+    atomicAdd(&bev[kBE_CROSS], (unsigned long long)n_cross);
+    __syncthreads();
+    if (P.events)
+        for (uint32_t i = threadIdx.x; i < kBE_COUNT; i += blockDim.x)
+            if (bev[i] != 0ull) atomicAdd(P.events + i, bev[i]);
+}
+
+}  // namespace pmo

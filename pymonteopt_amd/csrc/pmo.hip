@@ -11,6 +11,7 @@
 #include <mutex>
 
 #include "pmo.h"
+#include "batch.hpp"
 #include "walk.hpp"
 
 namespace {
@@ -131,8 +132,7 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     if (run->n_packets == 0) return PMO_OK;
     const int bs = run->block_size > 0 ? run->block_size : kDefaultBlock;
     if (bs % 64 != 0 || bs > 1024) return fail(PMO_ERR_INVALID, "block_size %d must be a multiple of 64, <= 1024", bs);
-    const int ppl = run->packets_per_lane > 0 ? run->packets_per_lane : kDefaultPacketsPerLane;
-    if (ppl > 4096) return fail(PMO_ERR_INVALID, "packets_per_lane %d too large", ppl);
+    if (run->packets_per_lane > 4096) return fail(PMO_ERR_INVALID, "packets_per_lane %d too large", run->packets_per_lane);
     if (tal->nr < 0 || tal->na < 0 || tal->nl < 0 || tal->n_rings < 0)
         return fail(PMO_ERR_INVALID, "negative histogram size");
     if (tal->nr > 0 && !(tal->dr > 0.f)) return fail(PMO_ERR_INVALID, "dr must be > 0");
@@ -161,7 +161,7 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     P.seed = run->seed; P.stream = run->stream;
     P.wth = run->wth; P.roulette_m = run->roulette_m; P.inv_m = 1.f / run->roulette_m;
     P.flags = run->flags;
-    P.batch = (uint32_t)(64 * ppl);
+    P.batch = 64u;  // set below once the resident grid is known
     P.max_events = kMaxEvents;
     P.scalars = tal->scalars;
     P.events = reinterpret_cast<unsigned long long*>(tal->events);
@@ -179,6 +179,7 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     P.ring_w = tal->n_rings > 0 ? tal->ring_w : nullptr;
     P.exit_state = tal->exit_state;
     P.work = reinterpret_cast<unsigned long long*>(workspace);
+    P.key_dev = run->key_dev;
 
     // LDS window of the absorbed tally: the launch voxel region, where deposits collide
     int wdim[3];
@@ -230,6 +231,15 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     if (rc != PMO_OK) return rc;
     unsigned long long grid = run->grid_blocks > 0 ? (unsigned long long)run->grid_blocks
                                                    : (unsigned long long)nb * (unsigned long long)d.n_cu;
+    // packets a wave claims per counter atomic: the caller's choice, else as many as keep at
+    // least four claims per resident wave (so small runs still spread over the whole grid)
+    int ppl = run->packets_per_lane;
+    if (ppl <= 0) {
+        const unsigned long long waves = grid * (unsigned long long)(bs / 64);
+        const unsigned long long per = run->n_packets / (64ull * 4ull * (waves > 0 ? waves : 1ull));
+        ppl = per < 1 ? 1 : (per > (unsigned long long)kDefaultPacketsPerLane ? kDefaultPacketsPerLane : (int)per);
+    }
+    P.batch = (uint32_t)(64 * ppl);
     const unsigned long long need_blocks = (run->n_packets + P.batch - 1) / P.batch;
     const unsigned long long waves_per_block = (unsigned long long)bs / 64;
     const unsigned long long useful = (need_blocks + waves_per_block - 1) / waves_per_block;
@@ -260,6 +270,93 @@ int pmo_philox(const uint32_t* ctr, uint32_t key0, uint32_t key1, uint32_t* out,
                        reinterpret_cast<const uint4*>(ctr), key0, key1, reinterpret_cast<uint4*>(out), n);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "philox launch");
+    return PMO_OK;
+}
+
+int64_t pmo_plate_schedule(const uint64_t* n_packets, int32_t n_inst, int32_t batch, uint32_t* sched, int64_t cap)
+{
+    g_err[0] = '\0';
+    if (!n_packets || n_inst < 1) return fail(PMO_ERR_INVALID, "need at least one instance");
+    if (batch < 1) return fail(PMO_ERR_INVALID, "batch must be >= 1");
+    if (cap > 0 && !sched) return fail(PMO_ERR_INVALID, "sched is null");
+    int64_t total = 0;
+    uint64_t max_rounds = 0;
+    for (int32_t i = 0; i < n_inst; ++i) {
+        if (n_packets[i] >= (1ull << 32)) return fail(PMO_ERR_UNSUPPORTED, "instance %d budget >= 2^32", i);
+        const uint64_t r = (n_packets[i] + (uint64_t)batch - 1) / (uint64_t)batch;
+        total += (int64_t)r;
+        if (r > max_rounds) max_rounds = r;
+    }
+    if (cap <= 0) return total;
+    int64_t m = 0;
+    for (uint64_t round = 0; round < max_rounds && m < cap; ++round) {
+        for (int32_t i = 0; i < n_inst && m < cap; ++i) {
+            const uint64_t first = round * (uint64_t)batch;
+            if (first >= n_packets[i]) continue;
+            const uint64_t left = n_packets[i] - first;
+            sched[4 * m + 0] = (uint32_t)i;
+            sched[4 * m + 1] = (uint32_t)first;
+            sched[4 * m + 2] = (uint32_t)(left < (uint64_t)batch ? left : (uint64_t)batch);
+            sched[4 * m + 3] = 0u;
+            ++m;
+        }
+    }
+    return total;
+}
+
+int pmo_walk_plates(const pmo_plate_instance* inst, int32_t n_inst, const uint32_t* schedule, int64_t n_batches,
+                    const pmo_run* run, const pmo_batch_tallies* tal, void* workspace, void* stream)
+{
+    g_err[0] = '\0';
+    static_assert(sizeof(pmo_plate_instance) == sizeof(pmo::PlateInstance), "instance layout");
+    if (!inst || !schedule || !run || !tal || !workspace) return fail(PMO_ERR_INVALID, "null argument");
+    if (n_inst < 1) return fail(PMO_ERR_INVALID, "n_inst must be >= 1");
+    if (n_batches < 0) return fail(PMO_ERR_INVALID, "n_batches < 0");
+    if (!tal->scalars) return fail(PMO_ERR_INVALID, "tallies.scalars is required");
+    if (tal->nr < 0 || (tal->nr > 0 && !(tal->dr > 0.f))) return fail(PMO_ERR_INVALID, "bad radial bins");
+    if (!(run->roulette_m >= 1.f)) return fail(PMO_ERR_INVALID, "roulette_m must be >= 1");
+    if ((run->flags & PMO_F_FP64) != 0u) return fail(PMO_ERR_UNSUPPORTED, "the batched walk runs in FP32 only");
+    if (n_batches == 0) return PMO_OK;
+    const int bs = run->block_size > 0 ? run->block_size : kDefaultBlock;
+    if (bs % 64 != 0 || bs > 1024) return fail(PMO_ERR_INVALID, "block_size %d must be a multiple of 64, <= 1024", bs);
+    DevInfo d;
+    int rc = device_info(&d);
+    if (rc != PMO_OK) return rc;
+
+    pmo::BatchParams P;
+    memset(&P, 0, sizeof(P));
+    P.inst = reinterpret_cast<const pmo::PlateInstance*>(inst);
+    P.n_inst = n_inst;
+    P.schedule = reinterpret_cast<const uint4*>(schedule);
+    P.n_batches = (unsigned long long)n_batches;
+    P.seed = run->seed; P.stream = run->stream;
+    P.key_dev = run->key_dev;
+    P.wth = run->wth; P.roulette_m = run->roulette_m; P.inv_m = 1.f / run->roulette_m;
+    P.flags = run->flags;
+    P.max_events = kMaxEvents;
+    P.scalars = tal->scalars;
+    P.events = reinterpret_cast<unsigned long long*>(tal->events);
+    P.nr = tal->nr; P.dr = tal->dr;
+    P.rd_r = tal->nr > 0 ? tal->rd_r : nullptr;
+    P.tt_r = tal->nr > 0 ? tal->tt_r : nullptr;
+    P.work = reinterpret_cast<unsigned long long*>(workspace);
+
+    int nb = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::plate_batch_kernel<float>, bs, 0);
+    if (e != hipSuccess) return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (nb < 1) nb = 1;
+    unsigned long long grid = run->grid_blocks > 0 ? (unsigned long long)run->grid_blocks
+                                                   : (unsigned long long)nb * (unsigned long long)d.n_cu;
+    const unsigned long long wpb = (unsigned long long)bs / 64;
+    const unsigned long long useful = ((unsigned long long)n_batches + wpb - 1) / wpb;
+    if (grid > useful) grid = useful;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(pmo::zero_u64_kernel, dim3(1), dim3(64), 0, s, P.work, 4);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "zero workspace launch");
+    hipLaunchKernelGGL(pmo::plate_batch_kernel<float>, dim3((unsigned)grid), dim3(bs), 0, s, P);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "batched walk launch");
     return PMO_OK;
 }
 

@@ -107,6 +107,7 @@ struct WalkParams {
     double* ring_w;
     float* exit_state;
     unsigned long long* work;
+    const uint32_t* key_dev;   // (seed, stream) in device memory, or null
     // LDS-privatised box of the absorbed tally around the launch voxel
     int32_t win_x0, win_y0, win_z0;
     uint32_t win_nx, win_ny, win_nz, win_cells;
@@ -146,9 +147,134 @@ __device__ __forceinline__ int bin_of(float x, float width, int nbins)
     return (b >= 0 && b < nbins) ? b : -1;
 }
 
+// ------------------------------------------------------------------ shared physics
+// These are the per-event formulas of pymopt/voxel/_kernel.py, shared by every walk
+// kernel so that a packet with the same draws takes the same path in each of them.
+
+template <typename R>
+__device__ __forceinline__ LabelConst<R> label_const(const float4 o)
+{
+    LabelConst<R> c;
+    const R ma = (R)o.x, ms = (R)o.y, g = (R)o.z;
+    c.mt = ma + ms;
+    c.ra = (c.mt > (R)0) ? ma / c.mt : (R)0;
+    c.n = (R)o.w;
+    c.g = g;
+    c.hg1 = (R)1 + g * g;
+    c.hg2 = (R)1 - g * g;
+    c.hg3 = (R)1 - g;
+    c.hg4 = (R)2 * g;
+    c.hginv = (g != (R)0) ? (R)1 / ((R)2 * g) : (R)0;
+    c.pad[0] = c.pad[1] = c.pad[2] = (R)0;
+    return c;
+}
+
+// Fresnel reflectance for |v_k| = ci going from index ni into nt (:84-93), in cosine form;
+// ct receives cos of the refraction angle. R = 0 at exactly normal incidence like the
+// reference (its ai == 0 branch) unless normal_limit asks for ((ni-nt)/(ni+nt))^2.
+template <typename R>
+__device__ __forceinline__ R fresnel_reflectance(R ni, R nt, R ci, bool normal_limit, R& ct)
+{
+    using M = Mth<R>;
+    ct = (R)1;
+    if (ci == (R)1) {
+        const R q = (ni - nt) / (ni + nt);
+        return normal_limit ? q * q : (R)0;
+    }
+    const R si = M::sqrt(M::fmax((R)1 - ci * ci, (R)0));
+    const R sa = ni / nt * si;
+    if (ni > nt && sa >= (R)1) return (R)1;        // total internal reflection (:85-86)
+    ct = M::sqrt(M::fmax((R)1 - sa * sa, (R)0));
+    const R sam = si * ct - ci * sa;               // sin(ai - at)
+    const R sap = si * ct + ci * sa;               // sin(ai + at)
+    const R cam = ci * ct + si * sa;               // cos(ai - at)
+    const R cap = ci * ct - si * sa;               // cos(ai + at)
+    const R q1 = M::div(sam, sap);
+    const R q2 = M::div(cap, cam);
+    return (R)0.5 * q1 * q1 * ((R)1 + q2 * q2);
+}
+
+// refraction into the next voxel along axis k (:97-106): tangential components scale by
+// ni/nt, the normal one becomes sign(v_k) cos(at), then renormalise
+template <typename R>
+__device__ __forceinline__ void refract(int k, R vk, R rn, R ct, R& vx, R& vy, R& vz)
+{
+    using M = Mth<R>;
+    R tx = vx * rn, ty = vy * rn, tz = vz * rn;
+    const R nk = M::copysign(ct, vk);
+    if (k == 0) tx = nk; else if (k == 1) ty = nk; else tz = nk;
+    const R inv = M::rsqrt(tx * tx + ty * ty + tz * tz);
+    vx = tx * inv; vy = ty * inv; vz = tz * inv;
+}
+
+// Henyey-Greenstein cos(theta) (:12-21), clamped to [-1, 1]
+template <typename R>
+__device__ __forceinline__ R hg_cos(const LabelConst<R>& L, R u)
+{
+    using M = Mth<R>;
+    if (L.g != (R)0) {
+        const R t = L.hg2 * M::rcp(L.hg3 + L.hg4 * u);
+        const R c = (L.hg1 - t * t) * L.hginv;
+        return c < (R)-1 ? (R)-1 : (c > (R)1 ? (R)1 : c);
+    }
+    return (R)2 * u - (R)1;
+}
+
+// new direction after scattering by theta (cos c) and azimuth 2 pi u_phi (:144-165)
+template <typename R>
+__device__ __forceinline__ void spin(R c, R u_phi, R& vx, R& vy, R& vz)
+{
+    using M = Mth<R>;
+    const R sn = M::sqrt(M::fmax((R)1 - c * c, (R)0));
+    R sf, cf;
+    M::sincos2pi(u_phi, &sf, &cf);
+    R nx_, ny_, nz_;
+    if (M::fabs(vz) > (R)0.99999) {
+        nx_ = sn * cf;
+        ny_ = sn * sf;
+        nz_ = vz > (R)0 ? c : -c;              // sign(v_z) * cos(theta), :153
+    } else {
+        const R den = M::sqrt((R)1 - vz * vz);
+        const R iden = M::rcp(den);
+        nx_ = sn * (vx * vz * cf - vy * sf) * iden + vx * c;
+        ny_ = sn * (vy * vz * cf + vx * sf) * iden + vy * c;
+        nz_ = -sn * cf * den + vz * c;
+    }
+    const R inv = M::rsqrt(nx_ * nx_ + ny_ * ny_ + nz_ * nz_);
+    vx = nx_ * inv; vy = ny_ * inv; vz = nz_ * inv;
+}
+
+// lateral launch position of packet (plo, phi) for the flat-disc (kind 1) and Gaussian
+// (kind 2) beams, as a voxel offset from the beam centre plus an in-voxel position
+template <typename R>
+__device__ __forceinline__ void launch_xy(int kind, float radius, R pitch, R half, uint32_t plo, uint32_t phi,
+                                          uint32_t seed, uint32_t stream, int nx, int ny, int32_t& ax,
+                                          int32_t& ay, R& px, R& py)
+{
+    using M = Mth<R>;
+    const uint4 r = philox4x32_10(make_uint4(0u, kTagSource, plo, phi), seed, stream);
+    const R u0 = u01<R>(r.x), u1 = u01<R>(r.y);
+    R rad;
+    if (kind == 1) rad = (R)radius * M::sqrt(u0);
+    else rad = (R)(radius * 0.5f) * M::sqrt((R)2 * M::neglog(u0));
+    R sa, ca;
+    M::sincos2pi(u1, &sa, &ca);
+    const R x = rad * ca, y = rad * sa;
+    const int dx = (int)__builtin_floor((double)(x / pitch + (R)0.5));
+    const int dy = (int)__builtin_floor((double)(y / pitch + (R)0.5));
+    ax += dx; ay += dy;
+    px = x - pitch * (R)dx;
+    py = y - pitch * (R)dy;
+    // keep the launch voxel inside the padded volume's interior
+    if (ax < 1) { ax = 1; px = -half; }
+    if (ax > nx - 2) { ax = nx - 2; px = half; }
+    if (ay < 1) { ay = 1; py = -half; }
+    if (ay > ny - 2) { ay = ny - 2; py = half; }
+}
+
 // ------------------------------------------------------------------ the walk kernel
 template <typename R>
-__global__ void __launch_bounds__(1024) walk_kernel(const WalkParams P)
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) walk_kernel(const WalkParams P)
 {
     using M = Mth<R>;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -159,22 +285,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(const WalkParams P)
     double* win = reinterpret_cast<double*>(smem + P.lds_win);
 
     const uint32_t end = P.n_labels;
-    for (uint32_t i = threadIdx.x; i <= end; i += blockDim.x) {
-        const float4 o = P.optics[i];
-        LabelConst<R> c;
-        const R ma = (R)o.x, ms = (R)o.y, g = (R)o.z;
-        c.mt = ma + ms;
-        c.ra = (c.mt > (R)0) ? ma / c.mt : (R)0;
-        c.n = (R)o.w;
-        c.g = g;
-        c.hg1 = (R)1 + g * g;
-        c.hg2 = (R)1 - g * g;
-        c.hg3 = (R)1 - g;
-        c.hg4 = (R)2 * g;
-        c.hginv = (g != (R)0) ? (R)1 / ((R)2 * g) : (R)0;
-        c.pad[0] = c.pad[1] = c.pad[2] = (R)0;
-        tab[i] = c;
-    }
+    for (uint32_t i = threadIdx.x; i <= end; i += blockDim.x) tab[i] = label_const<R>(P.optics[i]);
     for (uint32_t i = threadIdx.x; i < kBS_COUNT; i += blockDim.x) bscal[i] = 0.0;
     for (uint32_t i = threadIdx.x; i < kBE_COUNT; i += blockDim.x) bev[i] = 0ull;
     for (uint32_t i = threadIdx.x; i < P.h_total; i += blockDim.x) hist[i] = 0.0;
@@ -186,6 +297,8 @@ __global__ void __launch_bounds__(1024) walk_kernel(const WalkParams P)
     const R half = (R)P.pitch * (R)0.5;
     const R pitch = (R)P.pitch;
     const bool normal_limit = (P.flags & 2u) != 0u;
+    const uint32_t seed = P.key_dev ? __builtin_amdgcn_readfirstlane(P.key_dev[0]) : P.seed;
+    const uint32_t skey = P.key_dev ? __builtin_amdgcn_readfirstlane(P.key_dev[1]) : P.stream;
 
     // lane state
     bool alive = false;
@@ -229,26 +342,9 @@ __global__ void __launch_bounds__(1024) walk_kernel(const WalkParams P)
                     const uint32_t plo = (uint32_t)pid, phi = (uint32_t)(pid >> 32);
                     ax = P.ix; ay = P.iy; az = P.iz;
                     px = 0; py = 0; pz = -half;
-                    if (P.src_kind != 0) {
-                        const uint4 r = philox4x32_10(make_uint4(0u, kTagSource, plo, phi), P.seed, P.stream);
-                        const R u0 = u01<R>(r.x), u1 = u01<R>(r.y);
-                        R rad;
-                        if (P.src_kind == 1) rad = (R)P.radius * M::sqrt(u0);
-                        else rad = (R)(P.radius * 0.5f) * M::sqrt((R)2 * M::neglog(u0));
-                        R sa, ca;
-                        M::sincos2pi(u1, &sa, &ca);
-                        const R x = rad * ca, y = rad * sa;
-                        const int dx = (int)__builtin_floor((double)(x / pitch + (R)0.5));
-                        const int dy = (int)__builtin_floor((double)(y / pitch + (R)0.5));
-                        ax += dx; ay += dy;
-                        px = x - pitch * (R)dx;
-                        py = y - pitch * (R)dy;
-                        // keep the launch voxel inside the padded volume's interior
-                        if (ax < 1) { ax = 1; px = -half; }
-                        if (ax > (int)P.nx - 2) { ax = (int)P.nx - 2; px = half; }
-                        if (ay < 1) { ay = 1; py = -half; }
-                        if (ay > (int)P.ny - 2) { ay = (int)P.ny - 2; py = half; }
-                    }
+                    if (P.src_kind != 0)
+                        launch_xy<R>(P.src_kind, P.radius, pitch, half, plo, phi, seed, skey, (int)P.nx,
+                                     (int)P.ny, ax, ay, px, py);
                     const uint32_t vi = ((uint32_t)ax * P.ny + (uint32_t)ay) * P.nz + (uint32_t)az;
                     vx = (R)P.dirx; vy = (R)P.diry; vz = (R)P.dirz;
                     ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
@@ -258,7 +354,7 @@ __global__ void __launch_bounds__(1024) walk_kernel(const WalkParams P)
                     ni = tab[lab].n;
                     mt = tab[lab].mt;
                     inter = 0; fres = 0; nev = 0;
-                    const uint4 r = philox4x32_10(make_uint4(0u, kTagWalk, plo, phi), P.seed, P.stream);
+                    const uint4 r = philox4x32_10(make_uint4(0u, kTagWalk, plo, phi), seed, skey);
                     st = M::neglog(u01<R>(r.x));
                     u_roul = u01<R>(r.y); u_th = u01<R>(r.z); u_phi = u01<R>(r.w);
                     alive = true;
@@ -314,42 +410,16 @@ __global__ void __launch_bounds__(1024) walk_kernel(const WalkParams P)
                     R u;
                     if ((fres & 3u) == 0u) {
                         const uint4 r = philox4x32_10(make_uint4(fres >> 2, kTagFresnel, (uint32_t)pid,
-                                                                 (uint32_t)(pid >> 32)), P.seed, P.stream);
+                                                                 (uint32_t)(pid >> 32)), seed, skey);
                         u = u01<R>(r.x); fb0 = u01<R>(r.y); fb1 = u01<R>(r.z); fb2 = u01<R>(r.w);
                     } else {
                         u = fb0; fb0 = fb1; fb1 = fb2;
                     }
                     ++fres;
-                    const R ci = M::fabs(vk);
-                    R rf, ct = (R)1;
-                    if (ci == (R)1) {
-                        // normal incidence: the reference takes R = 0 (ai == 0 branch, :92-93)
-                        const R q = (ni - nt) / (ni + nt);
-                        rf = normal_limit ? q * q : (R)0;
-                    } else {
-                        const R si = M::sqrt(M::fmax((R)1 - ci * ci, (R)0));
-                        const R sa = ni / nt * si;
-                        if (ni > nt && sa >= (R)1) {
-                            rf = (R)1;  // total internal reflection (:85-86)
-                        } else {
-                            ct = M::sqrt(M::fmax((R)1 - sa * sa, (R)0));
-                            const R sam = si * ct - ci * sa;   // sin(ai - at)
-                            const R sap = si * ct + ci * sa;   // sin(ai + at)
-                            const R cam = ci * ct + si * sa;   // cos(ai - at)
-                            const R cap = ci * ct - si * sa;   // cos(ai + at)
-                            const R q1 = M::div(sam, sap);
-                            const R q2 = M::div(cap, cam);
-                            rf = (R)0.5 * q1 * q1 * ((R)1 + q2 * q2);
-                        }
-                    }
+                    R ct;
+                    const R rf = fresnel_reflectance<R>(ni, nt, M::fabs(vk), normal_limit, ct);
                     if (rf < u) {
-                        // refraction (:97-106)
-                        const R rn = ni / nt;
-                        R tx = vx * rn, ty = vy * rn, tz = vz * rn;
-                        const R nk = M::copysign(ct, vk);
-                        if (k == 0) tx = nk; else if (k == 1) ty = nk; else tz = nk;
-                        const R inv = M::rsqrt(tx * tx + ty * ty + tz * tz);
-                        vx = tx * inv; vy = ty * inv; vz = tz * inv;
+                        refract<R>(k, vk, ni / nt, ct, vx, vy, vz);
                         ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
                     } else {
                         transmit = false;
@@ -404,36 +474,11 @@ __global__ void __launch_bounds__(1024) walk_kernel(const WalkParams P)
                     }
                 }
                 if (!killed) {
-                    // Henyey-Greenstein spin (:143-165)
-                    R c;
-                    if (L.g != (R)0) {
-                        const R t = L.hg2 * M::rcp(L.hg3 + L.hg4 * u_th);
-                        c = (L.hg1 - t * t) * L.hginv;
-                        c = c < (R)-1 ? (R)-1 : (c > (R)1 ? (R)1 : c);
-                    } else {
-                        c = (R)2 * u_th - (R)1;
-                    }
-                    const R sn = M::sqrt(M::fmax((R)1 - c * c, (R)0));
-                    R sf, cf;
-                    M::sincos2pi(u_phi, &sf, &cf);
-                    R nx_, ny_, nz_;
-                    if (M::fabs(vz) > (R)0.99999) {
-                        nx_ = sn * cf;
-                        ny_ = sn * sf;
-                        nz_ = vz > (R)0 ? c : -c;              // sign(v_z) * cos(theta), :153
-                    } else {
-                        const R den = M::sqrt((R)1 - vz * vz);
-                        const R iden = M::rcp(den);
-                        nx_ = sn * (vx * vz * cf - vy * sf) * iden + vx * c;
-                        ny_ = sn * (vy * vz * cf + vx * sf) * iden + vy * c;
-                        nz_ = -sn * cf * den + vz * c;
-                    }
-                    const R inv = M::rsqrt(nx_ * nx_ + ny_ * ny_ + nz_ * nz_);
-                    vx = nx_ * inv; vy = ny_ * inv; vz = nz_ * inv;
+                    spin<R>(hg_cos<R>(L, u_th), u_phi, vx, vy, vz);      // (:143-165)
                     ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
                     ++inter;
                     const uint4 r = philox4x32_10(make_uint4(inter, kTagWalk, (uint32_t)pid,
-                                                             (uint32_t)(pid >> 32)), P.seed, P.stream);
+                                                             (uint32_t)(pid >> 32)), seed, skey);
                     st = M::neglog(u01<R>(r.x));
                     u_roul = u01<R>(r.y); u_th = u01<R>(r.z); u_phi = u01<R>(r.w);
                 } else {
