@@ -165,6 +165,23 @@ typedef struct pmo_batch_tallies {
     double* tt_r;            /* device [n_inst][nr] v_z > 0 exits, or NULL                       */
 } pmo_batch_tallies;
 
+/* ---- differentiable walk ------------------------------------------------------------ */
+typedef struct pmo_diff {
+    const float* mua_field;  /* device [nx*ny*nz] per-voxel mu_a replacing the label's, or NULL   */
+    int32_t nr;              /* radial bins of the detected signal Rd(r), (i*dr, (i+1)*dr]          */
+    float dr;
+    double* rd_r;            /* forward: device [nr] weight of v_z < 0 exits per bin               */
+    double* jac;             /* forward: device [nr][n_labels][2] sums of W L_j and W n_j of those
+                                exits (L_j path length, n_j interactions in label j), or NULL;
+                                needs n_labels <= 4                                                 */
+    int32_t* exit_rec;       /* device [2*n_packets] per packet (bin or -1, weight as float bits):
+                                written by the forward pass, read by the adjoint                    */
+    const double* coef;      /* adjoint: device [nr], d loss / d Rd_r[b] divided by (area_b * N)    */
+    double* grad_mua;        /* adjoint: device [nx*ny*nz] accumulates d loss / d mu_a per voxel    */
+    uint64_t* mismatch;      /* adjoint: device counter of replayed exits that differ from the
+                                forward record (0 when replay is exact), or NULL                    */
+} pmo_diff;
+
 /* ---- library ------------------------------------------------------------------------ */
 int pmo_abi_version(void);
 const char* pmo_last_error(void);
@@ -186,6 +203,21 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run,
  * 4*n uint32. */
 int pmo_philox(const uint32_t* ctr, uint32_t key0, uint32_t key1, uint32_t* out, int64_t n,
                void* stream);
+
+/* Forward pass of the differentiable walk: the walk of pmo_walk restricted to the scalar
+ * tallies (into `scalars` [PMO_S_COUNT], `events` [PMO_EV_COUNT] or NULL) and the detected
+ * signal Rd(r), plus the perturbation-MC sums of pmo_diff.jac and the per-packet exit
+ * record. No reference counterpart: the reference estimates optical properties by a
+ * parameter sweep (.ipynb_checkpoints/si_in_光学特性推定-checkpoint.ipynb:13945-13967). */
+int pmo_walk_forward_diff(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, double* scalars,
+                          uint64_t* events, const pmo_diff* diff, void* workspace, void* stream);
+
+/* Adjoint by path replay: regenerates from its Philox counter the path of every packet whose
+ * recorded exit bin has coef != 0 and accumulates -coef[bin] * W * (length in voxel v) into
+ * grad_mua[v]. Needs the same volume, source, run (seed, stream, packets) and mua_field as
+ * the forward pass that wrote exit_rec. */
+int pmo_walk_adjoint(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, const pmo_diff* diff,
+                     void* workspace, void* stream);
 
 /* Host helper: the round-robin batch schedule of a sweep. Writes up to `cap` batches
  * (instance, first local packet id, count, 0) into `sched` (host, 4*cap uint32), cycling over

@@ -71,6 +71,12 @@ class PmoBatchTallies(C.Structure):
                 ("rd_r", C.c_void_p), ("tt_r", C.c_void_p)]
 
 
+class PmoDiff(C.Structure):
+    _fields_ = [("mua_field", C.c_void_p), ("nr", C.c_int32), ("dr", C.c_float), ("rd_r", C.c_void_p),
+                ("jac", C.c_void_p), ("exit_rec", C.c_void_p), ("coef", C.c_void_p), ("grad_mua", C.c_void_p),
+                ("mismatch", C.c_void_p)]
+
+
 # every symbol include/pmo.h declares, with its ctypes signature
 SIGNATURES = {
     "pmo_abi_version": ([], C.c_int),
@@ -80,6 +86,10 @@ SIGNATURES = {
     "pmo_walk": ([C.POINTER(PmoVolume), C.POINTER(PmoSource), C.POINTER(PmoRun), C.POINTER(PmoTallies),
                   C.c_void_p, C.c_void_p], C.c_int),
     "pmo_philox": ([C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
+    "pmo_walk_forward_diff": ([C.POINTER(PmoVolume), C.POINTER(PmoSource), C.POINTER(PmoRun), C.c_void_p,
+                               C.c_void_p, C.POINTER(PmoDiff), C.c_void_p, C.c_void_p], C.c_int),
+    "pmo_walk_adjoint": ([C.POINTER(PmoVolume), C.POINTER(PmoSource), C.POINTER(PmoRun), C.POINTER(PmoDiff),
+                          C.c_void_p, C.c_void_p], C.c_int),
     "pmo_plate_schedule": ([C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int64], C.c_int64),
     "pmo_walk_plates": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.POINTER(PmoRun),
                          C.POINTER(PmoBatchTallies), C.c_void_p, C.c_void_p], C.c_int),

@@ -1,0 +1,193 @@
+"""Differentiable walk: d Rd(r) / d optical properties on the GPU.
+
+Two autograd paths, both behind torch.autograd.Function:
+- `rd_of_label_optics(mua, mus)` : per-label absorption and scattering. The forward pass
+  accumulates the perturbation-MC sums (libpmo pmo_walk_forward_diff), so backward is a
+  product with the Jacobian it already holds.
+- `rd_of_absorption_field(field)` : per-voxel absorption. Backward replays every detected
+  packet from its Philox counter and deposits the gradient along its path
+  (libpmo pmo_walk_adjoint); no path is stored.
+
+`fit_layers` uses the first path with Adam to recover layer optical properties from a
+measured Rd(r), the job the reference does with a ParameterGrid sweep of separate walks
+(.ipynb_checkpoints/si_in_光学特性推定-checkpoint.ipynb:13945-13967).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _abi
+from .geometry import Source
+from .runtime import Walker
+
+
+PmoDiff = _abi.PmoDiff
+
+
+def _p(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+class DiffWalk:
+    """A label volume ready for differentiable walks of `n_packets` packets with Rd(r)
+    detected in `nr` bins of `dr` mm."""
+
+    def __init__(self, labels, pitch, optics, source: Source, n_packets=1e7, nr=40, dr=0.1, device="cuda:0"):
+        self.walker = Walker(labels, pitch, optics, device=device)
+        self.device = self.walker.device
+        self.base_optics = np.array(optics, np.float32)
+        self.n_labels = self.base_optics.shape[0] - 1
+        self.source = source
+        self.n_packets = int(n_packets)
+        self.nr, self.dr = int(nr), float(dr)
+        self.area = torch.tensor([2 * np.pi * (i + 0.5) * self.dr ** 2 for i in range(self.nr)], dtype=torch.float64,
+                                 device=self.device)
+        self.lib = _abi.lib()
+        self.last_mismatch = 0
+
+    @property
+    def shape(self):
+        return self.walker.shape
+
+    def _structs(self, seed, stream_id):
+        v = self.walker.volume_struct()
+        s = _abi.PmoSource()
+        src = self.source
+        s.kind = int(src.kind)
+        s.ix, s.iy, s.iz = (int(x) for x in src.voxel)
+        s.radius = float(src.radius)
+        for i in range(3):
+            s.dir[i] = float(src.dir[i])
+        s.w0 = float(src.w0)
+        r = _abi.PmoRun()
+        r.n_packets = self.n_packets
+        r.seed = int(seed) & 0xFFFFFFFF
+        r.stream = int(stream_id) & 0xFFFFFFFF
+        r.wth, r.roulette_m = 1e-4, 10.0
+        return v, s, r
+
+    def set_optics(self, optics):
+        self.walker.set_optics(np.asarray(optics, np.float32))
+
+    def forward(self, seed=0, stream_id=0, mua_field=None, jac=True):
+        """One forward pass. Returns Rd(r) per unit area and packet, the Jacobians
+        dRd(r)/dmu_a and dRd(r)/dmu_s per label (when jac), and the exit record."""
+        if jac and self.n_labels > 4:
+            raise ValueError("per-label Jacobians support at most 4 labels")
+        f64 = dict(dtype=torch.float64, device=self.device)
+        scal = torch.zeros(_abi.PMO_S_COUNT, **f64)
+        rd = torch.zeros(self.nr, **f64)
+        J = torch.zeros(self.nr, self.n_labels, 2, **f64) if jac else None
+        rec = torch.empty(2 * self.n_packets, dtype=torch.int32, device=self.device)
+        field = None
+        if mua_field is not None:
+            field = mua_field.detach().to(self.device, torch.float32).contiguous()
+            if tuple(field.shape) != tuple(self.shape):
+                raise ValueError(f"mua_field must have the padded volume shape {self.shape}")
+        d = PmoDiff(_p(field), self.nr, self.dr, _p(rd), _p(J), _p(rec), None, None, None)
+        v, s, r = self._structs(seed, stream_id)
+        with torch.cuda.device(self.device):
+            rc = self.lib.pmo_walk_forward_diff(C.byref(v), C.byref(s), C.byref(r), _p(scal), None, C.byref(d),
+                                                _p(self.walker.workspace),
+                                                C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream))
+        _abi.check(rc)
+        norm = self.area * self.n_packets
+        out = {"Rd_r": rd / norm, "scalars": scal, "exit_rec": rec, "field": field, "seed": seed,
+               "stream": stream_id}
+        if jac:
+            mus = torch.as_tensor(self.walker.optics[: self.n_labels, 1].to(torch.float64))
+            wl, wn = J[..., 0], J[..., 1]
+            out["J_mua"] = -wl / norm[:, None]
+            out["J_mus"] = (wn / mus[None, :] - wl) / norm[:, None]
+        return out
+
+    def adjoint(self, fwd, grad_rd_r):
+        """d loss / d mu_a per voxel (padded shape) for the forward pass `fwd`, given
+        d loss / d Rd(r)."""
+        g = grad_rd_r.detach().to(self.device, torch.float64).reshape(-1)
+        coef = (g / (self.area * self.n_packets)).contiguous()
+        grad = torch.zeros(self.shape, dtype=torch.float64, device=self.device)
+        mism = torch.zeros(1, dtype=torch.int64, device=self.device)
+        d = PmoDiff(_p(fwd["field"]), self.nr, self.dr, None, None, _p(fwd["exit_rec"]), _p(coef), _p(grad),
+                    _p(mism))
+        v, s, r = self._structs(fwd["seed"], fwd["stream"])
+        with torch.cuda.device(self.device):
+            rc = self.lib.pmo_walk_adjoint(C.byref(v), C.byref(s), C.byref(r), C.byref(d),
+                                           _p(self.walker.workspace),
+                                           C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream))
+        _abi.check(rc)
+        self.last_mismatch = mism
+        return grad
+
+
+class _LabelOptics(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, mua, mus, dw: DiffWalk, seed, stream_id):
+        opt = dw.base_optics.copy()
+        L = dw.n_labels
+        opt[:L, 0] = mua.detach().cpu().numpy()
+        opt[:L, 1] = mus.detach().cpu().numpy()
+        dw.set_optics(opt)
+        out = dw.forward(seed=seed, stream_id=stream_id, jac=True)
+        ctx.save_for_backward(out["J_mua"], out["J_mus"])
+        return out["Rd_r"].to(mua.dtype)
+
+    @staticmethod
+    def backward(ctx, g):
+        J_mua, J_mus = ctx.saved_tensors
+        g = g.to(torch.float64)
+        return (J_mua.T @ g).to(g.dtype), (J_mus.T @ g).to(g.dtype), None, None, None
+
+
+class _AbsorptionField(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, field, dw: DiffWalk, seed, stream_id):
+        out = dw.forward(seed=seed, stream_id=stream_id, mua_field=field, jac=False)
+        ctx.dw = dw
+        ctx.fwd = out
+        return out["Rd_r"].to(field.dtype)
+
+    @staticmethod
+    def backward(ctx, g):
+        grad = ctx.dw.adjoint(ctx.fwd, g)
+        return grad.to(g.dtype), None, None, None
+
+
+def rd_of_label_optics(mua, mus, dw: DiffWalk, seed=0, stream_id=0):
+    """Rd(r) [nr] as a differentiable function of per-label mu_a [L] and mu_s [L] (device tensors)."""
+    return _LabelOptics.apply(mua, mus, dw, seed, stream_id)
+
+
+def rd_of_absorption_field(field, dw: DiffWalk, seed=0, stream_id=0):
+    """Rd(r) [nr] as a differentiable function of a per-voxel mu_a field (padded shape)."""
+    return _AbsorptionField.apply(field, dw, seed, stream_id)
+
+
+def fit_layers(dw: DiffWalk, target_rd_r, g, mua0, musp0, iters=150, lr=0.05, seed0=1000, bins=None,
+               callback=None):
+    """Adam on log mu_a and log mu_s' of each label so that the walk's Rd(r) matches
+    `target_rd_r` (relative squared error over `bins`); g stays fixed, mu_s = mu_s'/(1-g).
+    Each iteration walks with a fresh seed."""
+    dev = dw.device
+    g = torch.as_tensor(np.asarray(g, np.float64), device=dev)
+    la = torch.log(torch.as_tensor(np.asarray(mua0, np.float64), device=dev)).requires_grad_(True)
+    ls = torch.log(torch.as_tensor(np.asarray(musp0, np.float64), device=dev)).requires_grad_(True)
+    target = torch.as_tensor(np.asarray(target_rd_r, np.float64), device=dev)
+    sel = torch.arange(target.numel(), device=dev) if bins is None else torch.as_tensor(bins, device=dev)
+    opt = torch.optim.Adam([la, ls], lr=lr)
+    hist = []
+    for it in range(iters):
+        opt.zero_grad()
+        mua = torch.exp(la)
+        mus = torch.exp(ls) / (1 - g)
+        rd = rd_of_label_optics(mua, mus, dw, seed=seed0 + it)
+        loss = (((rd[sel] - target[sel]) / target[sel]) ** 2).mean()
+        loss.backward()
+        opt.step()
+        hist.append(float(loss.detach()))
+        if callback:
+            callback(it, float(loss), torch.exp(la).detach().cpu().numpy(), torch.exp(ls).detach().cpu().numpy())
+    return torch.exp(la).detach().cpu().numpy(), torch.exp(ls).detach().cpu().numpy(), hist

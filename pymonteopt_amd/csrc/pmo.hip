@@ -11,6 +11,7 @@
 #include <mutex>
 
 #include "pmo.h"
+#include "adjoint.hpp"
 #include "batch.hpp"
 #include "walk.hpp"
 
@@ -271,6 +272,142 @@ int pmo_philox(const uint32_t* ctr, uint32_t key0, uint32_t key1, uint32_t* out,
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "philox launch");
     return PMO_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int diff_launch(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, double* scalars,
+                uint64_t* events, const pmo_diff* diff, void* workspace, void* stream, int mode)
+{
+    g_err[0] = '\0';
+    if (!vol || !src || !run || !diff || !workspace) return fail(PMO_ERR_INVALID, "null argument");
+    if (!vol->labels || !vol->optics) return fail(PMO_ERR_INVALID, "volume labels/optics pointer is null");
+    if (vol->nx < 3 || vol->ny < 3 || vol->nz < 3) return fail(PMO_ERR_INVALID, "volume extents must be >= 3");
+    const unsigned long long nvox = (unsigned long long)vol->nx * vol->ny * vol->nz;
+    if (nvox >= (1ull << 31) || vol->nx >= (1 << 24) || vol->ny >= (1 << 24))
+        return fail(PMO_ERR_UNSUPPORTED, "volume too large");
+    if (vol->n_labels < 1 || vol->n_labels > 255) return fail(PMO_ERR_LABEL, "n_labels outside [1, 255]");
+    if (!(vol->pitch > 0.f)) return fail(PMO_ERR_INVALID, "pitch must be > 0");
+    if (src->kind < PMO_BEAM_PENCIL || src->kind > PMO_BEAM_GAUSS) return fail(PMO_ERR_INVALID, "unknown beam kind");
+    if (src->ix < 1 || src->ix > vol->nx - 2 || src->iy < 1 || src->iy > vol->ny - 2 || src->iz < 1 ||
+        src->iz > vol->nz - 2)
+        return fail(PMO_ERR_INVALID, "launch voxel is not inside the volume interior");
+    if (!(run->roulette_m >= 1.f)) return fail(PMO_ERR_INVALID, "roulette_m must be >= 1");
+    if ((run->flags & PMO_F_FP64) != 0u) return fail(PMO_ERR_UNSUPPORTED, "the differentiable walk runs in FP32");
+    if (run->first_packet != 0) return fail(PMO_ERR_UNSUPPORTED, "first_packet must be 0");
+    if (run->n_packets >= (1ull << 32)) return fail(PMO_ERR_UNSUPPORTED, "n_packets >= 2^32");
+    if (diff->nr < 1 || !(diff->dr > 0.f)) return fail(PMO_ERR_INVALID, "need nr >= 1 radial bins and dr > 0");
+    if (mode == 0 && (!scalars || !diff->rd_r)) return fail(PMO_ERR_INVALID, "forward needs scalars and rd_r");
+    if (diff->jac && vol->n_labels > pmo::kMaxJacLabels)
+        return fail(PMO_ERR_UNSUPPORTED, "per-label sums support at most %d labels", pmo::kMaxJacLabels);
+    if (mode == 1 && (!diff->exit_rec || !diff->coef || !diff->grad_mua))
+        return fail(PMO_ERR_INVALID, "adjoint needs exit_rec, coef and grad_mua");
+    if (run->n_packets == 0) return PMO_OK;
+    const int bs = run->block_size > 0 ? run->block_size : kDefaultBlock;
+    if (bs % 64 != 0 || bs > 1024) return fail(PMO_ERR_INVALID, "block_size must be a multiple of 64, <= 1024");
+    DevInfo d;
+    int rc = device_info(&d);
+    if (rc != PMO_OK) return rc;
+
+    pmo::DiffParams P;
+    memset(&P, 0, sizeof(P));
+    P.labels = vol->labels;
+    P.optics = reinterpret_cast<const float4*>(vol->optics);
+    P.mua_field = diff->mua_field;
+    P.nx = (uint32_t)vol->nx; P.ny = (uint32_t)vol->ny; P.nz = (uint32_t)vol->nz;
+    P.n_labels = (uint32_t)vol->n_labels;
+    P.cx = (vol->nx - 1) / 2; P.cy = (vol->ny - 1) / 2;
+    P.pitch = vol->pitch;
+    P.src_kind = src->kind; P.ix = src->ix; P.iy = src->iy; P.iz = src->iz;
+    P.radius = src->radius; P.dirx = src->dir[0]; P.diry = src->dir[1]; P.dirz = src->dir[2];
+    P.w0 = src->w0;
+    P.n_packets = run->n_packets;
+    P.seed = run->seed; P.stream = run->stream;
+    P.wth = run->wth; P.roulette_m = run->roulette_m; P.inv_m = 1.f / run->roulette_m;
+    P.flags = run->flags;
+    P.max_events = kMaxEvents;
+    P.mode = mode;
+    P.scalars = scalars;
+    P.events = reinterpret_cast<unsigned long long*>(events);
+    P.nr = diff->nr; P.dr = diff->dr;
+    P.rd_r = diff->rd_r;
+    P.jac = mode == 0 ? diff->jac : nullptr;
+    P.exit_rec = reinterpret_cast<int2*>(diff->exit_rec);
+    P.coef = diff->coef;
+    P.grad = diff->grad_mua;
+    P.mismatch = reinterpret_cast<unsigned long long*>(diff->mismatch);
+    P.work = reinterpret_cast<unsigned long long*>(workspace);
+    if (mode == 1 && !P.mismatch) P.mismatch = P.work + 2;   // scratch counter in the workspace
+    // replay: LDS box of the gradient map around the launch voxel (same as the absorbed tally)
+    if (mode == 1) {
+        int wdim[3] = {kDefaultWindow[0], kDefaultWindow[1], kDefaultWindow[2]};
+        const int ext[3] = {vol->nx, vol->ny, vol->nz};
+        const int ctr[3] = {src->ix, src->iy, src->iz};
+        int org[3];
+        for (int a = 0; a < 3; ++a) {
+            if (wdim[a] > ext[a]) wdim[a] = ext[a];
+            org[a] = a < 2 ? ctr[a] - wdim[a] / 2 : ctr[a];
+            if (org[a] < 0) org[a] = 0;
+            if (org[a] + wdim[a] > ext[a]) org[a] = ext[a] - wdim[a];
+        }
+        P.win_x0 = org[0]; P.win_y0 = org[1]; P.win_z0 = org[2];
+        P.win_nx = (uint32_t)wdim[0]; P.win_ny = (uint32_t)wdim[1]; P.win_nz = (uint32_t)wdim[2];
+    }
+    P.win_cells = P.win_nx * P.win_ny * P.win_nz;
+    const uint32_t njac = P.jac ? P.n_labels : 0u;
+    P.lds_tab = 0;
+    P.lds_scal = (uint32_t)align_up((size_t)(vol->n_labels + 1) * sizeof(pmo::LabelConst<float>), 16);
+    P.lds_ev = P.lds_scal + (uint32_t)align_up(pmo::kBS_COUNT * sizeof(double), 16);
+    P.lds_hist = P.lds_ev + (uint32_t)align_up(pmo::kBE_COUNT * sizeof(unsigned long long), 16);
+    P.lds_jac = P.lds_hist + (uint32_t)align_up((size_t)P.nr * sizeof(double), 16);
+    P.lds_win = P.lds_jac + (uint32_t)align_up((size_t)P.nr * njac * 2u * sizeof(double), 16);
+    const size_t lds = P.lds_win + (size_t)P.win_cells * sizeof(double);
+    if (lds > kMaxLds) return fail(PMO_ERR_UNSUPPORTED, "differentiable walk needs %zu B of LDS", lds);
+
+    int nb = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::diff_kernel<float>, bs, lds);
+    if (e != hipSuccess) return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (nb < 1) nb = 1;
+    unsigned long long grid = run->grid_blocks > 0 ? (unsigned long long)run->grid_blocks
+                                                   : (unsigned long long)nb * (unsigned long long)d.n_cu;
+    int ppl = run->packets_per_lane;
+    if (ppl <= 0) {
+        const unsigned long long waves = grid * (unsigned long long)(bs / 64);
+        const unsigned long long per = run->n_packets / (64ull * 4ull * (waves > 0 ? waves : 1ull));
+        ppl = per < 1 ? 1 : (per > (unsigned long long)kDefaultPacketsPerLane ? kDefaultPacketsPerLane : (int)per);
+    }
+    P.batch = (uint32_t)(64 * ppl);
+    const unsigned long long need_blocks = (run->n_packets + P.batch - 1) / P.batch;
+    const unsigned long long wpb = (unsigned long long)bs / 64;
+    const unsigned long long useful = (need_blocks + wpb - 1) / wpb;
+    if (grid > useful) grid = useful;
+    if (grid < 1) grid = 1;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(pmo::zero_u64_kernel, dim3(1), dim3(64), 0, s, P.work, 4);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "zero workspace launch");
+    hipLaunchKernelGGL(pmo::diff_kernel<float>, dim3((unsigned)grid), dim3(bs), lds, s, P);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "differentiable walk launch");
+    return PMO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pmo_walk_forward_diff(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, double* scalars,
+                          uint64_t* events, const pmo_diff* diff, void* workspace, void* stream)
+{
+    return diff_launch(vol, src, run, scalars, events, diff, workspace, stream, 0);
+}
+
+int pmo_walk_adjoint(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, const pmo_diff* diff,
+                     void* workspace, void* stream)
+{
+    return diff_launch(vol, src, run, nullptr, nullptr, diff, workspace, stream, 1);
 }
 
 int64_t pmo_plate_schedule(const uint64_t* n_packets, int32_t n_inst, int32_t batch, uint32_t* sched, int64_t cap)

@@ -64,7 +64,8 @@ template <typename R> struct LabelConst {
     R hg3;     // 1 - g
     R hg4;     // 2 g
     R hginv;   // 1 / (2 g)
-    R pad[3];
+    R ms;      // mu_s
+    R pad[2];
 };
 
 // ------------------------------------------------------------------ kernel parameters
@@ -165,7 +166,8 @@ __device__ __forceinline__ LabelConst<R> label_const(const float4 o)
     c.hg3 = (R)1 - g;
     c.hg4 = (R)2 * g;
     c.hginv = (g != (R)0) ? (R)1 / ((R)2 * g) : (R)0;
-    c.pad[0] = c.pad[1] = c.pad[2] = (R)0;
+    c.ms = ms;
+    c.pad[0] = c.pad[1] = (R)0;
     return c;
 }
 

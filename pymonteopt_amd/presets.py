@@ -73,3 +73,21 @@ def bone_workload(seed=0, n=256) -> Workload:
     rings = tuple((0.25 * i - 0.125, 0.25 * i + 0.125) for i in range(1, 11))
     return Workload(f"bone phantom {n}^3 @0.02mm", vol, 0.02, opt, src,
                     dict(nr=50, dr=0.05, rings=rings))
+
+
+def grf_grid(seed=0, n=64, pitch=0.1, corr_len=0.3) -> Workload:
+    """Heterogeneous 4-label grid: n^3 voxels at `pitch` whose interior labels are the
+    quartiles of a Gaussian random field (correlation length `corr_len` mm); pencil beam
+    at the top-face centre; Rd(r) in 40 bins of 0.1 mm."""
+    from .geometry import gaussian_random_field, shell
+    f = gaussian_random_field((n - 2, n - 2, n - 2), pitch, corr_len, seed)
+    q = np.quantile(f, [0.25, 0.5, 0.75])
+    vol = np.empty((n, n, n), np.uint8)
+    vol[1:-1, 1:-1, 1:-1] = np.digitize(f, q).astype(np.uint8)
+    shell(vol, 4)
+    opt = optics_table(ma=[0.02, 0.05, 0.1, 0.2], ms=[10.0, 15.0, 20.0, 25.0], g=[0.9, 0.9, 0.85, 0.8],
+                       n=[1.37, 1.40, 1.44, 1.40], n_ambient=1.0)
+    top = int(vol[center_add(n), center_add(n), 1])
+    w0 = float(np.float32(1.0) - np.float32(specular(1.0, float(opt[top, 3]))))
+    src = Source(voxel=(center_add(n), center_add(n), 1), w0=w0)
+    return Workload(f"GRF 4-label grid {n}^3 @{pitch}mm", vol, pitch, opt, src, dict(nr=40, dr=0.1))

@@ -37,7 +37,8 @@ PROBE = r"""
 #define S(t) printf(#t " %zu\n", sizeof(t));
 #define O(t, f) printf(#t "." #f " %zu\n", offsetof(t, f));
 int main(void) {
-  S(pmo_volume) S(pmo_source) S(pmo_run) S(pmo_tallies)
+  S(pmo_volume) S(pmo_source) S(pmo_run) S(pmo_tallies) S(pmo_plate_instance) S(pmo_batch_tallies) S(pmo_diff)
+  O(pmo_diff, mismatch) O(pmo_plate_instance, n_packets) O(pmo_run, key_dev)
   O(pmo_volume, optics) O(pmo_source, w0) O(pmo_run, packets_per_lane) O(pmo_tallies, exit_state)
   O(pmo_tallies, ring_w) O(pmo_tallies, line_z) O(pmo_run, flags) O(pmo_run, window)
   printf("PMO_S_COUNT %d\nPMO_EV_COUNT %d\n", PMO_S_COUNT, PMO_EV_COUNT);
@@ -54,13 +55,18 @@ def test_ctypes_layout_matches_header(tmp_path):
     out = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], capture_output=True, text=True,
                                                               check=True).stdout.strip().split("\n"))
     for name, cls in (("pmo_volume", _abi.PmoVolume), ("pmo_source", _abi.PmoSource),
-                      ("pmo_run", _abi.PmoRun), ("pmo_tallies", _abi.PmoTallies)):
+                      ("pmo_run", _abi.PmoRun), ("pmo_tallies", _abi.PmoTallies),
+                      ("pmo_plate_instance", _abi.PmoPlateInstance), ("pmo_batch_tallies", _abi.PmoBatchTallies),
+                      ("pmo_diff", _abi.PmoDiff)):
         assert int(out[name]) == C.sizeof(cls), name
     assert int(out["pmo_volume.optics"]) == _abi.PmoVolume.optics.offset
     assert int(out["pmo_source.w0"]) == _abi.PmoSource.w0.offset
     assert int(out["pmo_run.packets_per_lane"]) == _abi.PmoRun.packets_per_lane.offset
     assert int(out["pmo_run.flags"]) == _abi.PmoRun.flags.offset
     assert int(out["pmo_run.window"]) == _abi.PmoRun.window.offset
+    assert int(out["pmo_run.key_dev"]) == _abi.PmoRun.key_dev.offset
+    assert int(out["pmo_diff.mismatch"]) == _abi.PmoDiff.mismatch.offset
+    assert int(out["pmo_plate_instance.n_packets"]) == _abi.PmoPlateInstance.n_packets.offset
     assert int(out["pmo_tallies.exit_state"]) == _abi.PmoTallies.exit_state.offset
     assert int(out["pmo_tallies.ring_w"]) == _abi.PmoTallies.ring_w.offset
     assert int(out["pmo_tallies.line_z"]) == _abi.PmoTallies.line_z.offset
