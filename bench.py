@@ -27,18 +27,149 @@ def cpu_baseline(wl, budget_s=12.0):
     og.build()
     ma, ms, g, n = wl.optics_columns()
     threads = len(os.sched_getaffinity(0))
-    probe = 2000
-    t0 = time.perf_counter()
-    og.walk(wl.labels, wl.pitch, ma, ms, g, n, wl.source.as_dict(), probe, 12345, precision="f32",
-            threads=threads, nr=40, dr=0.1)
-    rate = probe / (time.perf_counter() - t0)
-    n_s = int(max(probe, min(5e6, rate * budget_s)))
-    t0 = time.perf_counter()
-    og.walk(wl.labels, wl.pitch, ma, ms, g, n, wl.source.as_dict(), n_s, 777, precision="f32",
-            threads=threads, nr=40, dr=0.1)
-    dt = time.perf_counter() - t0
+    kw = dict(precision="f32", threads=threads, nr=40, dr=0.1)
+    n_s, dt = 2000, 0.0
+    while True:                          # grow the sample until it takes a few seconds
+        t0 = time.perf_counter()
+        og.walk(wl.labels, wl.pitch, ma, ms, g, n, wl.source.as_dict(), n_s, 777, **kw)
+        dt = time.perf_counter() - t0
+        if dt > 2.0 or n_s >= 5_000_000:
+            break
+        n_s *= 4
+    if dt < budget_s / 2 and n_s < 5_000_000:
+        n_s = int(min(5_000_000, n_s * budget_s / dt))
+        t0 = time.perf_counter()
+        og.walk(wl.labels, wl.pitch, ma, ms, g, n, wl.source.as_dict(), n_s, 778, **kw)
+        dt = time.perf_counter() - t0
     return {"value": n_s / dt, "unit": "photons/s", "cores": threads, "kind": "port",
             "sample": f"{n_s} packets of the same workload (f32 oracle, absorbed + Rd(r) tallies), {dt:.1f} s"}
+
+
+def _events():
+    import torch
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def batched_leg(n_inst=512):
+    """512 random slabs (uneven budgets) in one graph-captured persistent launch, against
+    512 sequential plain walks of the same plates with the same draws (so each instance's
+    estimate, and its standard error, is the same in both)."""
+    import numpy as np
+    import torch
+    from pymonteopt_amd import autotune
+    from pymonteopt_amd.batched import PlateSweep, slab_sweep
+    from pymonteopt_amd.geometry import Source
+    from pymonteopt_amd.runtime import TallySpec, Walker
+    plates = slab_sweep(n_inst, seed=0)
+    N = sum(p.n_packets for p in plates)
+    sw = PlateSweep(plates, nr=40, dr=0.1, batch=256)
+    tuned = autotune.tune_sweep(sw, blocks=(128, 256), batches=(128, 256, 512))
+    sw.capture()
+    sw.replay(1)
+    torch.cuda.synchronize()
+    e0, e1 = _events()
+    e0.record()
+    sw.replay(2)
+    e1.record()
+    e1.synchronize()
+    tb = e0.elapsed_time(e1) * 1e-3
+    rb = sw.results()
+    walkers = []
+    for p in plates:
+        kind, vox, w0 = p.source()
+        walkers.append((Walker(p.volume(), p.voxel_space, p.optics()), Source(voxel=vox, w0=w0, kind=kind)))
+    spec = TallySpec(nr=40, dr=0.1, absorbed=False)
+    tals = [w.tallies(spec) for w, _ in walkers]
+    torch.cuda.synchronize()
+    e0.record()
+    for i, ((w, s), t) in enumerate(zip(walkers, tals)):
+        w.walk(s, plates[i].n_packets, 2, t, stream_id=i)
+    e1.record()
+    e1.synchronize()
+    tp = e0.elapsed_time(e1) * 1e-3
+    rd_p = np.array([t.to_host()["scalars"]["Rd"] / p.n_packets for t, p in zip(tals, plates)])
+    return {"instances": n_inst, "packets": N, "batched_s": tb, "batched_photons_per_s": N / tb,
+            "sequential_plain_s": tp, "sequential_photons_per_s": N / tp, "speedup": tp / tb,
+            "max_abs_Rd_diff_same_draws": float(np.max(np.abs(rd_p - rb["Rd"]))),
+            "tuned": {k: v for k, v in tuned.items() if k != "table"}, "graph": True}
+
+
+def adjoint_leg(n_packets=10_000_000):
+    """Differentiable walk at 1e7 packets: the skin plate's forward with per-label
+    Jacobians, and forward + replay adjoint of the per-voxel absorption on the 64^3
+    4-label grid, each against the plain walk of the same packets."""
+    import numpy as np
+    import torch
+    from pymonteopt_amd import presets
+    from pymonteopt_amd.adjoint import DiffWalk
+    from pymonteopt_amd.runtime import TallySpec, Walker
+    out = {}
+
+    def timed(fn):
+        fn()
+        e0, e1 = _events()
+        e0.record()
+        r = fn()
+        e1.record()
+        e1.synchronize()
+        return r, e0.elapsed_time(e1) * 1e-3
+
+    for name, wl in (("skin", presets.skin_plate()), ("grf64", presets.grf_grid(seed=0))):
+        dw = DiffWalk(wl.labels, wl.pitch, wl.optics, wl.source, n_packets=n_packets, nr=40, dr=0.1)
+        wk = Walker(wl.labels, wl.pitch, wl.optics)
+        tal = wk.tallies(TallySpec(nr=40, dr=0.1, absorbed=False))
+        _, t_plain = timed(lambda: wk.walk(wl.source, n_packets, 3, tal))
+        field = None
+        if name == "grf64":
+            mua = torch.tensor(wl.optics[:, 0], device="cuda:0")
+            field = mua[torch.as_tensor(np.minimum(wl.labels, wl.n_labels), device="cuda:0").long()].float()
+        fwd, t_fwd = timed(lambda: dw.forward(seed=3, mua_field=field, jac=True))
+        _, t_adj = timed(lambda: dw.adjoint(fwd, dw.area.clone()))
+        out[name] = {"plain_walk_s": t_plain, "forward_with_jacobian_s": t_fwd, "adjoint_replay_s": t_adj,
+                     "gradient_cost_vs_plain": (t_fwd + t_adj) / t_plain,
+                     "replay_mismatches": int(dw.last_mismatch.item())}
+    return out
+
+
+def measure_peaks():
+    """Re-measures the two peaks with libpmo's probes: FP64 global atomic adds per second
+    (pseudo-random addresses in 32 MiB) and FP32 FMA lane-ops per second."""
+    import ctypes as C
+    import torch
+    from pymonteopt_amd import _abi
+    L = _abi.lib()
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    buf = torch.zeros(1 << 22, dtype=torch.float64, device="cuda:0")
+    blocks, iters = 4096, 256
+    out = {}
+    for _ in range(2):
+        e0, e1 = _events()
+        e0.record()
+        _abi.check(L.pmo_probe_atomics(C.c_void_p(buf.data_ptr()), buf.numel(), blocks, iters, s))
+        e1.record()
+        e1.synchronize()
+        out["fp64_atomics_per_s"] = blocks * 256 * iters / (e0.elapsed_time(e1) * 1e-3)
+    vb, vi = 2048, 8192
+    o = torch.empty(vb * 256, dtype=torch.float32, device="cuda:0")
+    for _ in range(2):
+        e0, e1 = _events()
+        e0.record()
+        _abi.check(L.pmo_probe_valu(C.c_void_p(o.data_ptr()), vb, vi, s))
+        e1.record()
+        e1.synchronize()
+        out["valu_lane_ops_per_s"] = vb * 256.0 * vi * 32 / (e0.elapsed_time(e1) * 1e-3)
+    return out
+
+
+def traffic_from_profile():
+    """HBM bytes per walk launch measured by rocprofv3 (profiles/walk_skin_traffic.json)."""
+    path = os.path.join(ROOT, "profiles", "walk_skin_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path))
+    except ValueError:
+        return None
 
 
 def main():
@@ -54,6 +185,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-absorbed", action="store_true")
     ap.add_argument("--window", default="0,0,0", help="LDS tally window x,y,z (0 = default, -1 = none)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched and adjoint legs")
+    ap.add_argument("--quorum", type=int, default=0, help="lanes ready before a wave runs interactions")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
@@ -74,7 +207,7 @@ def main():
     tal = wk.tallies(spec)
     N = int(args.packets)
     kw = dict(block_size=args.block, packets_per_lane=args.ppl, fp64=args.fp64,
-              window=tuple(int(x) for x in args.window.split(",")))
+              window=tuple(int(x) for x in args.window.split(",")), quorum=args.quorum)
 
     for i in range(args.warmup):
         tal.zero()
@@ -160,6 +293,30 @@ def main():
             "hbm_algorithmic_GBps": N * bytes_pp / mean_launch_s / 1e9,
         },
     }
+    peaks = measure_peaks()
+    glob_pp = ev_sum.get("global_deposits", 0) / (N * args.steps)
+    # VALU peak re-measured by libpmo's FMA probe (non-packed FP32); the nominal figure
+    # 256 CU x 64 lanes x 2.4 GHz is kept beside it
+    out["roofline"]["peak_nominal"] = out["roofline"]["peak"]
+    out["roofline"]["peak"] = peaks["valu_lane_ops_per_s"] / 1e12
+    out["roofline"]["frac"] = achieved / peaks["valu_lane_ops_per_s"]
+    out["roofline"]["frac_of_nominal_peak"] = achieved / roofline.PEAK_VALU_LANE_OPS
+    out["roofline_atomic"] = {
+        "bound": "fp64_atomic",
+        "achieved": glob_pp * N / mean_launch_s,
+        "peak": peaks["fp64_atomics_per_s"],
+        "unit": "atomics/s",
+        "frac": glob_pp * N / mean_launch_s / peaks["fp64_atomics_per_s"],
+        "global_deposits_per_packet": glob_pp,
+    }
+    prof = traffic_from_profile()
+    if prof is not None and int(prof.get("packets_per_launch", 0)) > 0:
+        scale = N / float(prof["packets_per_launch"])
+        out["roofline"]["traffic"] = prof["hbm_bytes_per_launch"] * scale
+        out["roofline"]["traffic_source"] = prof["source"]
+    if not args.no_extra:
+        out["batched"] = batched_leg()
+        out["adjoint"] = adjoint_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl)
     print(json.dumps(out), flush=True)

@@ -78,7 +78,8 @@ enum {
     PMO_EV_ROULETTE = 4, /* roulette tests (w <= wth)                                         */
     PMO_EV_KILLED = 5,   /* packets ended by roulette                                         */
     PMO_EV_ESCAPED = 6,  /* packets that entered an end (ambient) voxel                       */
-    PMO_EV_COUNT = 7
+    PMO_EV_GLOBAL_DEPOSITS = 7, /* FP64 global atomics issued for the absorbed-weight map        */
+    PMO_EV_COUNT = 8
 };
 
 /* ---- geometry store ----------------------------------------------------------------- */
@@ -116,6 +117,8 @@ typedef struct pmo_run {
                                starting at its z; {0,0,0} = default, {-1,-1,-1} = none          */
     const uint32_t* key_dev; /* device (seed, stream) read by the kernel instead of seed/stream when
                                 non-NULL, so a captured HIP graph can be replayed with new keys */
+    int32_t quorum;          /* a wave runs its interaction step once this many of its 64 lanes are
+                                ready for one (else it advances face crossings); 0 = default     */
 } pmo_run;
 
 typedef struct pmo_tallies {
@@ -218,6 +221,14 @@ int pmo_walk_forward_diff(const pmo_volume* vol, const pmo_source* src, const pm
  * the forward pass that wrote exit_rec. */
 int pmo_walk_adjoint(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, const pmo_diff* diff,
                      void* workspace, void* stream);
+
+/* Diagnostics that re-measure the two peaks the walk is judged against (no reference
+ * counterpart). The caller times them on `stream` (e.g. with HIP events):
+ * pmo_probe_atomics issues blocks*256*iters FP64 atomic adds (no return) to pseudo-random
+ * addresses of the device array buf[n]; pmo_probe_valu runs blocks*256 lanes that each
+ * issue iters x 32 independent FP32 FMAs and write one float to out[blocks*256]. */
+int pmo_probe_atomics(double* buf, int64_t n, int32_t blocks, int32_t iters, void* stream);
+int pmo_probe_valu(float* out, int32_t blocks, int32_t iters, void* stream);
 
 /* Host helper: the round-robin batch schedule of a sweep. Writes up to `cap` batches
  * (instance, first local packet id, count, 0) into `sched` (host, 4*cap uint32), cycling over

@@ -211,6 +211,11 @@ static int FN(packet)(const og_volume* V, const og_source* S, const og_run* R, c
         a[0] += dx; a[1] += dy;
         p[0] = x - l * (REAL)dx;
         p[1] = y - l * (REAL)dy;
+        /* rounding of x / l can put the position a hair outside the voxel: clamp */
+        if (p[0] < -half) p[0] = -half;
+        if (p[0] > half) p[0] = half;
+        if (p[1] < -half) p[1] = -half;
+        if (p[1] > half) p[1] = half;
     }
     REAL w = (REAL)S->w0;
     if (C) { C->scal[6] += (double)w; C->ev[0]++; }

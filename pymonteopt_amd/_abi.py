@@ -27,8 +27,9 @@ PMO_F_NORMAL_FRESNEL = 1 << 1
 SCALAR_NAMES = ["Rd", "Tt", "other", "A", "kill", "gain", "w0",
                 "face_x-", "face_x+", "face_y-", "face_y+", "face_z-", "face_z+"]
 PMO_S_COUNT = 13
-EVENT_NAMES = ["packets", "interactions", "crossings", "fresnel", "roulette", "killed", "escaped"]
-PMO_EV_COUNT = 7
+EVENT_NAMES = ["packets", "interactions", "crossings", "fresnel", "roulette", "killed", "escaped",
+               "global_deposits"]
+PMO_EV_COUNT = 8
 
 
 class PmoVolume(C.Structure):
@@ -46,7 +47,7 @@ class PmoRun(C.Structure):
                 ("seed", C.c_uint32), ("stream", C.c_uint32),
                 ("wth", C.c_float), ("roulette_m", C.c_float), ("flags", C.c_uint32),
                 ("block_size", C.c_int32), ("grid_blocks", C.c_int32), ("packets_per_lane", C.c_int32),
-                ("window", C.c_int32 * 3), ("key_dev", C.c_void_p)]
+                ("window", C.c_int32 * 3), ("key_dev", C.c_void_p), ("quorum", C.c_int32)]
 
 
 class PmoTallies(C.Structure):
@@ -90,6 +91,8 @@ SIGNATURES = {
                                C.c_void_p, C.POINTER(PmoDiff), C.c_void_p, C.c_void_p], C.c_int),
     "pmo_walk_adjoint": ([C.POINTER(PmoVolume), C.POINTER(PmoSource), C.POINTER(PmoRun), C.POINTER(PmoDiff),
                           C.c_void_p, C.c_void_p], C.c_int),
+    "pmo_probe_atomics": ([C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p], C.c_int),
+    "pmo_probe_valu": ([C.c_void_p, C.c_int32, C.c_int32, C.c_void_p], C.c_int),
     "pmo_plate_schedule": ([C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int64], C.c_int64),
     "pmo_walk_plates": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.POINTER(PmoRun),
                          C.POINTER(PmoBatchTallies), C.c_void_p, C.c_void_p], C.c_int),

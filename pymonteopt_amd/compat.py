@@ -1,0 +1,69 @@
+"""Kernel-level drop-in for pyMonteOpt's walk functions.
+
+`vmc_kernel` has the signature and return value of pymopt/voxel/_kernel.py:23-28 (the Numba
+walk, also the argument list of the CuPy launch in pymopt/voxel_gpu/_classes.py:105-111),
+so a pymopt model can run on the MI355X by swapping one import. It reads the launch state
+the model prepared (add, p, v, w of every packet), runs libpmo's walk on the GPU and returns
+per-packet exit states that the model's own _encooder / getRdTtRate
+(pymopt/voxel/_classes.py:261-293) turn into the same results.
+
+Launch states the device walk generates itself are accepted: all packets identical
+(pencil beam, optionally oblique). Per-packet launch arrays of other shapes (the reference's
+Gaussian beam) are rejected; use the model API's w_beam instead.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _abi
+from .geometry import Source
+from .runtime import TallySpec, Walker
+
+__all__ = ["vmc_kernel"]
+
+
+def vmc_kernel(add, p, v, w, ma, ms, n, g, voxel_model, l, nPh, end_point, rand_seed=None):
+    """Returns (add, p, v, w) after the walk, in the layout of the reference kernel:
+    exit voxel address and in-voxel position (encoded so that l*(add - centre) + p is the
+    exit point), exit direction, exit weight; zeros for packets ended by roulette."""
+    add = np.asarray(add)
+    p = np.asarray(p, np.float32)
+    v = np.asarray(v, np.float32)
+    w = np.asarray(w, np.float32)
+    N = int(nPh)
+    if add.shape != (3, N) or p.shape != (3, N) or v.shape != (3, N) or w.shape != (N,):
+        raise ValueError("add, p, v must be (3, nPh) and w (nPh,)")
+    for arr in (add, p, v):
+        if not np.all(arr == arr[:, :1]):
+            raise ValueError("per-packet launch states differ; only identical launches are generated on the device")
+    if not np.all(w == w[0]):
+        raise ValueError("per-packet launch weights differ")
+    vm = np.ascontiguousarray(voxel_model).astype(np.uint8)
+    end = int(end_point)
+    nl = int(np.asarray(ma).size)
+    if end != nl:
+        raise ValueError("end_point must equal the number of tissue labels (the label tables' length)")
+    opt = np.zeros((nl + 1, 4), np.float32)
+    opt[:nl, 0] = np.asarray(ma, np.float32)
+    opt[:nl, 1] = np.asarray(ms, np.float32)
+    opt[:nl, 2] = np.asarray(g, np.float32)
+    opt[:, 3] = np.asarray(n, np.float32)[: nl + 1]
+    if not np.isclose(float(p[2, 0]), -float(l) / 2, rtol=1e-5) or p[0, 0] != 0 or p[1, 0] != 0:
+        raise ValueError("launch position must be the centre of the voxel's top face (0, 0, -l/2)")
+    src = Source(voxel=tuple(int(x) for x in add[:, 0]), w0=float(w[0]),
+                 dir=tuple(float(x) for x in v[:, 0]))
+    seed = np.random.randint(1e5) if rand_seed is None else int(rand_seed)
+    wk = Walker(vm, float(l), opt)
+    tal = wk.tallies(TallySpec(absorbed=False, exit_state=True), N)
+    wk.walk(src, N, seed, tal)
+    es = tal.to_host()["exit_state"]
+    x, y, z, vx, vy, vz, ww = es
+    cx, cy = (vm.shape[0] - 1) // 2, (vm.shape[1] - 1) // 2
+    add_o = np.empty((3, N), np.int32)
+    add_o[0], add_o[1], add_o[2] = cx, cy, 1
+    p_o = np.stack([x, y, z - np.float32(l) / 2]).astype(np.float32)
+    v_o = np.stack([vx, vy, vz]).astype(np.float32)
+    killed = ww == 0
+    add_o[:, killed] = 0
+    p_o[:, killed] = 0
+    return add_o, p_o, v_o, ww.astype(np.float32)

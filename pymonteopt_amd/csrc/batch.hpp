@@ -165,9 +165,10 @@ plate_batch_kernel(const BatchParams P)
             ended = true; xs = vz > (R)0 ? 1 : -1;
         } else {
             ++nev;
-            const R d0 = (half - (vx > (R)0 ? px : -px)) * ivx;
-            const R d1 = (half - (vy > (R)0 ? py : -py)) * ivy;
-            const R d2 = (half - (vz > (R)0 ? pz : -pz)) * ivz;
+            // only components with |v_i| > 0 take part (_kernel.py:65), as in the reference
+            const R d0 = vx != (R)0 ? (half - (vx > (R)0 ? px : -px)) * ivx : (R)INFINITY;
+            const R d1 = vy != (R)0 ? (half - (vy > (R)0 ? py : -py)) * ivy : (R)INFINITY;
+            const R d2 = vz != (R)0 ? (half - (vz > (R)0 ? pz : -pz)) * ivz : (R)INFINITY;
             R db = (R)1000;
             int k = 0;
             if (d0 < db) { db = d0; k = 0; }

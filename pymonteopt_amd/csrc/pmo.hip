@@ -36,9 +36,10 @@ int hip_fail(hipError_t e, const char* what)
 constexpr size_t kWorkspaceBytes = 256;
 constexpr int kDefaultBlock = 256;
 constexpr int kDefaultPacketsPerLane = 4;
-constexpr uint32_t kMaxEvents = 1u << 28;
+constexpr uint32_t kMaxEvents = 1u << 24;
 constexpr size_t kMaxLds = 64 * 1024;
 constexpr int kDefaultWindow[3] = {16, 16, 8};
+constexpr int kDefaultQuorum = 48;
 
 struct DevInfo {
     int n_cu = 0;
@@ -164,6 +165,9 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     P.flags = run->flags;
     P.batch = 64u;  // set below once the resident grid is known
     P.max_events = kMaxEvents;
+    if (run->quorum < 0 || run->quorum > 64) return fail(PMO_ERR_INVALID, "quorum %d outside [0, 64]", run->quorum);
+    // >= 1, so a wave whose lanes all wait for crossings still advances them
+    P.quorum = run->quorum > 0 ? (uint32_t)run->quorum : (uint32_t)kDefaultQuorum;
     P.scalars = tal->scalars;
     P.events = reinterpret_cast<unsigned long long*>(tal->events);
     P.absorbed = tal->absorbed;
@@ -258,6 +262,26 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "walk launch");
     return PMO_OK;
+}
+
+int pmo_probe_atomics(double* buf, int64_t n, int32_t blocks, int32_t iters, void* stream)
+{
+    g_err[0] = '\0';
+    if (!buf || n < 1 || n >= (1ll << 32) || blocks < 1 || iters < 1) return fail(PMO_ERR_INVALID, "bad probe size");
+    hipLaunchKernelGGL(pmo::probe_atomics_kernel, dim3(blocks), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       buf, (uint32_t)n, iters);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PMO_OK : hip_fail(e, "probe launch");
+}
+
+int pmo_probe_valu(float* out, int32_t blocks, int32_t iters, void* stream)
+{
+    g_err[0] = '\0';
+    if (!out || blocks < 1 || iters < 1) return fail(PMO_ERR_INVALID, "bad probe size");
+    hipLaunchKernelGGL(pmo::probe_valu_kernel, dim3(blocks), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       out, iters);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PMO_OK : hip_fail(e, "probe launch");
 }
 
 int pmo_philox(const uint32_t* ctr, uint32_t key0, uint32_t key1, uint32_t* out, int64_t n, void* stream)

@@ -87,6 +87,7 @@ struct WalkParams {
     uint32_t flags;
     uint32_t batch;            // packet ids claimed per wave per counter atomic
     uint32_t max_events;       // per-packet event guard
+    uint32_t quorum;           // lanes ready for an interaction before a wave runs that step
     // tallies
     double* scalars;
     unsigned long long* events;
@@ -119,7 +120,8 @@ struct WalkParams {
 
 // block-level accumulator slots in LDS (doubles)
 enum { kBS_RD = 0, kBS_TT, kBS_OTHER, kBS_ABS, kBS_KILL, kBS_GAIN, kBS_W0, kBS_FACE, kBS_COUNT = 13 };
-enum { kBE_PACKETS = 0, kBE_INTERACT, kBE_CROSS, kBE_FRESNEL, kBE_ROULETTE, kBE_KILLED, kBE_ESCAPED, kBE_COUNT = 7 };
+enum { kBE_PACKETS = 0, kBE_INTERACT, kBE_CROSS, kBE_FRESNEL, kBE_ROULETTE, kBE_KILLED, kBE_ESCAPED,
+       kBE_GLOBAL_DEPOSITS, kBE_COUNT = 8 };
 
 __device__ __forceinline__ void lds_add(double* p, double v) { atomicAdd(p, v); }
 __device__ __forceinline__ void lds_add(unsigned long long* p, unsigned long long v) { atomicAdd(p, v); }
@@ -133,11 +135,15 @@ __device__ __forceinline__ void gmem_add(double* p, double v)
 // of the global tally, slot < 0 is -(cell + 1) of the block's LDS window
 constexpr int32_t kNoSlot = 0x7fffffff;
 
-__device__ __forceinline__ void flush_deposit(const WalkParams& P, double* win, int32_t slot, double v)
+__device__ __forceinline__ void flush_deposit(const WalkParams& P, double* win, int32_t slot, double v, uint32_t& n_glob)
 {
     if (slot == kNoSlot) return;
-    if (slot < 0) lds_add(win + (-slot - 1), v);
-    else gmem_add(P.absorbed + slot, v);
+    if (slot < 0) {
+        lds_add(win + (-slot - 1), v);
+    } else {
+        gmem_add(P.absorbed + slot, v);
+        ++n_glob;
+    }
 }
 
 __device__ __forceinline__ int bin_of(float x, float width, int nbins)
@@ -265,8 +271,11 @@ __device__ __forceinline__ void launch_xy(int kind, float radius, R pitch, R hal
     const int dx = (int)__builtin_floor((double)(x / pitch + (R)0.5));
     const int dy = (int)__builtin_floor((double)(y / pitch + (R)0.5));
     ax += dx; ay += dy;
+    // in-voxel position, clamped: rounding of x / pitch can put it a hair outside the voxel
     px = x - pitch * (R)dx;
     py = y - pitch * (R)dy;
+    px = px < -half ? -half : (px > half ? half : px);
+    py = py < -half ? -half : (py > half ? half : py);
     // keep the launch voxel inside the padded volume's interior
     if (ax < 1) { ax = 1; px = -half; }
     if (ax > nx - 2) { ax = nx - 2; px = half; }
@@ -314,7 +323,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
     double acc_abs = 0.0;
     int32_t dep_slot = kNoSlot;      // where the pending deposit goes (see flush_deposit)
     double dep_acc = 0.0;
-    uint32_t n_cross = 0, n_pk = 0;
+    uint32_t n_cross = 0, n_pk = 0, n_glob = 0;
 
     // wave-uniform work batch
     unsigned long long b_next = 0, b_end = 0;
@@ -366,6 +375,33 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
             }
         }
         if (__ballot(alive) == 0ull) break;
+
+        // ------------------------------------------------------------ classify
+        // Every live lane finds its next event: a face crossing if the remaining optical
+        // depth reaches the exit face of its voxel, else the end of the hop inside it.
+        // The wave then runs one of the two steps (a wave-uniform choice): the interaction
+        // step once `quorum` lanes are ready for it or no lane has a crossing left, else
+        // the crossing step. Lanes not served wait with their state unchanged, so each
+        // packet's own sequence of events, and its draws, are the same as in any order.
+        R db = (R)1000, smt = (R)0;
+        int k = 0;
+        bool want_int = false;
+        const bool walking = alive && lab != end;
+        if (walking) {
+            // distance to the exit face, x < y < z on ties (_kernel.py:63-68)
+            // only components with |v_i| > 0 take part (_kernel.py:65), as in the reference
+            const R d0 = vx != (R)0 ? (half - (vx > (R)0 ? px : -px)) * ivx : (R)INFINITY;
+            const R d1 = vy != (R)0 ? (half - (vy > (R)0 ? py : -py)) * ivy : (R)INFINITY;
+            const R d2 = vz != (R)0 ? (half - (vz > (R)0 ? pz : -pz)) * ivz : (R)INFINITY;
+            if (d0 < db) { db = d0; k = 0; }
+            if (d1 < db) { db = d1; k = 1; }
+            if (d2 < db) { db = d2; k = 2; }
+            smt = db * mt;
+            want_int = !(st >= smt);                               // (:69)
+        }
+        const unsigned long long m_int = __ballot(walking && want_int);
+        const unsigned long long m_cross = __ballot(walking && !want_int);
+        const bool run_int = m_cross == 0ull || (uint32_t)__popcll(m_int) >= P.quorum;
         if (!alive) continue;
 
         // ------------------------------------------------------------ one event
@@ -375,19 +411,9 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
         if (lab == end) {
             // launched inside an end voxel: leaves at once
             ended = true; xk = 2; xs = vz > (R)0 ? 1 : -1;
-        } else {
+        } else if (want_int == run_int) {
             ++nev;
-            // distance to the exit face, x < y < z on ties (_kernel.py:63-68)
-            const R d0 = (half - (vx > (R)0 ? px : -px)) * ivx;
-            const R d1 = (half - (vy > (R)0 ? py : -py)) * ivy;
-            const R d2 = (half - (vz > (R)0 ? pz : -pz)) * ivz;
-            R db = (R)1000;
-            int k = 0;
-            if (d0 < db) { db = d0; k = 0; }
-            if (d1 < db) { db = d1; k = 1; }
-            if (d2 < db) { db = d2; k = 2; }
-            const R smt = db * mt;
-            if (st >= smt) {
+            if (!want_int) {
                 // ------------------------------------------------ face crossing (:69-120)
                 ++n_cross;
                 px += vx * db; py += vy * db; pz += vz * db;
@@ -460,7 +486,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
                     if (slot == dep_slot) {
                         dep_acc += dep;
                     } else {
-                        flush_deposit(P, win, dep_slot, dep_acc);
+                        flush_deposit(P, win, dep_slot, dep_acc, n_glob);
                         dep_slot = slot;
                         dep_acc = dep;
                     }
@@ -556,11 +582,12 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
     }
 
     // ---------------------------------------------------------------- block flush
-    if (P.absorbed) flush_deposit(P, win, dep_slot, dep_acc);
+    if (P.absorbed) flush_deposit(P, win, dep_slot, dep_acc, n_glob);
     lds_add(&bscal[kBS_ABS], acc_abs);
     lds_add(&bscal[kBS_W0], (double)n_pk * (double)P.w0);
     lds_add(&bev[kBE_CROSS], (unsigned long long)n_cross);
     lds_add(&bev[kBE_PACKETS], (unsigned long long)n_pk);
+    lds_add(&bev[kBE_GLOBAL_DEPOSITS], (unsigned long long)n_glob);
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < kBS_COUNT; i += blockDim.x)
         if (bscal[i] != 0.0) gmem_add(P.scalars + i, bscal[i]);
@@ -594,6 +621,35 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n)
 {
     const int i = threadIdx.x;
     if (i < n) p[i] = 0ull;
+}
+
+// probes of the FP64 atomic rate and the FP32 FMA issue rate (every index in bounds)
+__global__ void probe_atomics_kernel(double* buf, uint32_t n, int iters)
+{
+    uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int i = 0; i < iters; ++i) {
+        x ^= x << 13; x ^= x >> 17; x ^= x << 5;          // xorshift32
+        gmem_add(buf + (x % n), 1.0);
+    }
+}
+
+__global__ void probe_valu_kernel(float* out, int iters)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    float a[8], b = 1.0000001f, c = 1e-7f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[j] = (float)(t + j);
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) a[j] = __builtin_fmaf(a[j], b, c);
+        }
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s += a[j];
+    out[t] = s;
 }
 
 __global__ void philox_kernel(const uint4* ctr, uint32_t k0, uint32_t k1, uint4* out, int64_t n)

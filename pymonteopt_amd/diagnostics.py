@@ -1,0 +1,56 @@
+"""Measurement helpers run from the command line.
+
+    python -m pymonteopt_amd.diagnostics copy     # 3 copies of 256 MiB (counter calibration)
+    python -m pymonteopt_amd.diagnostics peaks    # FP64 atomic rate and FP32 FMA issue rate
+
+`copy` gives rocprofv3's FETCH_SIZE / WRITE_SIZE a kernel of known traffic (read 256 MiB,
+write 256 MiB per dispatch), so walk-kernel counts can be scaled to bytes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import sys
+
+
+def copy_calibration(mib=256, reps=3):
+    import torch
+    n = mib * 1024 * 1024 // 4
+    a = torch.ones(n, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return {"bytes_read_per_dispatch": n * 4, "bytes_written_per_dispatch": n * 4, "dispatches": reps}
+
+
+def peaks():
+    import torch
+    from . import _abi
+    L = _abi.lib()
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def timed(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        fn()
+        e0.record()
+        fn()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) * 1e-3
+
+    out = {}
+    for mib in (0.25, 2, 32, 256):
+        buf = torch.zeros(int(mib * 1024 * 1024 // 8), dtype=torch.float64, device="cuda:0")
+        t = timed(lambda: _abi.check(L.pmo_probe_atomics(C.c_void_p(buf.data_ptr()), buf.numel(), 4096, 256, s)))
+        out[f"fp64_atomics_per_s_{mib}MiB"] = 4096 * 256 * 256 / t
+    o = torch.empty(2048 * 256, dtype=torch.float32, device="cuda:0")
+    t = timed(lambda: _abi.check(L.pmo_probe_valu(C.c_void_p(o.data_ptr()), 2048, 8192, s)))
+    out["valu_fp32_fma_lane_ops_per_s"] = 2048 * 256.0 * 8192 * 32 / t
+    return out
+
+
+if __name__ == "__main__":
+    what = sys.argv[1] if len(sys.argv) > 1 else "peaks"
+    res = copy_calibration() if what == "copy" else peaks()
+    print(json.dumps(res), flush=True)

@@ -154,7 +154,7 @@ class Walker:
 
     def walk(self, src: Source, n_packets: int, seed: int, tallies: TallySet, *, stream_id=0,
              first_packet=0, wth=1e-4, roulette_m=10.0, fp64=False, normal_fresnel=False,
-             block_size=0, grid_blocks=0, packets_per_lane=0, window=(0, 0, 0), hip_stream=None):
+             block_size=0, grid_blocks=0, packets_per_lane=0, window=(0, 0, 0), quorum=0, hip_stream=None):
         """Launch one walk (asynchronous on torch's current stream unless given)."""
         if tallies.exit_state is not None and tallies.n_exit != int(n_packets):
             raise ValueError("exit_state buffer was sized for a different packet count")
@@ -179,6 +179,7 @@ class Walker:
         r.packets_per_lane = int(packets_per_lane)
         for i in range(3):
             r.window[i] = int(window[i])
+        r.quorum = int(quorum)
         t = tallies.c_struct()
         if hip_stream is None:
             hip_stream = torch.cuda.current_stream(self.device).cuda_stream

@@ -38,7 +38,7 @@ PROBE = r"""
 #define O(t, f) printf(#t "." #f " %zu\n", offsetof(t, f));
 int main(void) {
   S(pmo_volume) S(pmo_source) S(pmo_run) S(pmo_tallies) S(pmo_plate_instance) S(pmo_batch_tallies) S(pmo_diff)
-  O(pmo_diff, mismatch) O(pmo_plate_instance, n_packets) O(pmo_run, key_dev)
+  O(pmo_diff, mismatch) O(pmo_plate_instance, n_packets) O(pmo_run, key_dev) O(pmo_run, quorum)
   O(pmo_volume, optics) O(pmo_source, w0) O(pmo_run, packets_per_lane) O(pmo_tallies, exit_state)
   O(pmo_tallies, ring_w) O(pmo_tallies, line_z) O(pmo_run, flags) O(pmo_run, window)
   printf("PMO_S_COUNT %d\nPMO_EV_COUNT %d\n", PMO_S_COUNT, PMO_EV_COUNT);
@@ -65,6 +65,7 @@ def test_ctypes_layout_matches_header(tmp_path):
     assert int(out["pmo_run.flags"]) == _abi.PmoRun.flags.offset
     assert int(out["pmo_run.window"]) == _abi.PmoRun.window.offset
     assert int(out["pmo_run.key_dev"]) == _abi.PmoRun.key_dev.offset
+    assert int(out["pmo_run.quorum"]) == _abi.PmoRun.quorum.offset
     assert int(out["pmo_diff.mismatch"]) == _abi.PmoDiff.mismatch.offset
     assert int(out["pmo_plate_instance.n_packets"]) == _abi.PmoPlateInstance.n_packets.offset
     assert int(out["pmo_tallies.exit_state"]) == _abi.PmoTallies.exit_state.offset

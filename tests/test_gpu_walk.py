@@ -222,3 +222,73 @@ def test_tally_window_placement_is_invisible(slab):
             assert r["scalars"][k] == pytest.approx(base["scalars"][k], rel=1e-12)
     with pytest.raises(Exception):
         gpu_run(wk, wl, N, 11, window=(64, 64, 32))
+
+
+def test_event_scheduling_is_invisible(slab):
+    """Which step a wave runs first (quorum) changes only the interleaving of lanes, not
+    any packet's own event sequence: tallies agree to FP64 summation order."""
+    wl, wk = slab
+    N = 50000
+    base = gpu_run(wk, wl, N, 13, quorum=1)
+    for q in (16, 48, 64):
+        r = gpu_run(wk, wl, N, 13, quorum=q)
+        assert r["events"] == base["events"]
+        assert np.allclose(r["absorbed"], base["absorbed"], rtol=1e-10, atol=1e-15)
+        for k in ("Rd", "Tt", "A"):
+            assert r["scalars"][k] == pytest.approx(base["scalars"][k], rel=1e-12)
+
+
+def test_compat_vmc_kernel_is_a_drop_in(slab):
+    """compat.vmc_kernel takes the reference kernel's arguments and returns exit states
+    that the reference's own post-processing (_encooder, getRdTtRate) turns into the
+    tallies of the same walk."""
+    from pymonteopt_amd.compat import vmc_kernel
+    wl, wk = slab
+    N = 30000
+    vm = wl.labels.astype(np.int8)
+    add = np.zeros((3, N), np.int32)
+    add[0], add[1], add[2] = wl.source.voxel
+    p = np.zeros((3, N), np.float32)
+    p[2] = -wl.pitch / 2
+    v = np.zeros((3, N), np.float32)
+    v[2] = 1
+    w = np.full(N, wl.source.w0, np.float32)
+    ma, ms, g, n = wl.optics_columns()
+    a2, p2, v2, w2 = vmc_kernel(add, p, v, w, ma, ms, n, g, vm, wl.pitch, N, 1, rand_seed=21)
+    ref = gpu_run(wk, wl, N, 21)
+    # getRdTtRate (pymopt/voxel/_classes.py:285-293)
+    assert w2[v2[2] < 0].sum(dtype=np.float64) == pytest.approx(ref["scalars"]["Rd"], rel=1e-6)
+    assert w2[v2[2] > 0].sum(dtype=np.float64) == pytest.approx(ref["scalars"]["Tt"], rel=1e-6)
+    # _encooder (pymopt/voxel/_classes.py:261-269) gives the exit points the tallies binned
+    cx, cy = (vm.shape[0] - 1) // 2, (vm.shape[1] - 1) // 2
+    x = wl.pitch * (a2[0] - cx) + p2[0]
+    y = wl.pitch * (a2[1] - cy) + p2[1]
+    rd = v2[2] < 0
+    r = np.sqrt(x[rd] ** 2 + y[rd] ** 2)
+    h, _ = np.histogram(r, bins=np.arange(51) * 0.1, weights=w2[rd].astype(np.float64))
+    assert np.allclose(h, ref["rd_r"], rtol=1e-5, atol=1e-6)
+    with pytest.raises(ValueError):
+        p3 = p.copy()
+        p3[0, 5] = 0.01
+        vmc_kernel(add, p3, v, w, ma, ms, n, g, vm, wl.pitch, N, 1)
+
+
+def test_flat_beam_launches_on_voxel_faces_finish(torch_cuda):
+    """Regression: a flat-beam launch position rounded a hair outside its voxel used to
+    meet a zero direction component in the boundary search, turn the position into NaN
+    and bounce between lateral faces until the event guard. Zero components now take no
+    part (as in the reference) and launch positions are clamped into the voxel."""
+    import time
+    from pymonteopt_amd import presets
+    from pymonteopt_amd.runtime import TallySpec, Walker
+    wl = presets.skin_plate("flat")
+    wk = Walker(wl.labels, wl.pitch, wl.optics)
+    N = 200000
+    for s in range(10):
+        t0 = time.time()
+        r = gpu_run(wk, wl, N, s, fp64=(s % 2 == 1))
+        assert time.time() - t0 < 5.0, s
+        ev = r["events"]
+        assert ev["escaped"] + ev["killed"] == N
+        assert (ev["interactions"] + ev["crossings"]) / N < 5000
+        assert ledger_error(r["scalars"]) < 1e-6
