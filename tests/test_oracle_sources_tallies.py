@@ -1,0 +1,127 @@
+"""Pins the oracle's launch sampling and exit tallies against their definitions.
+
+* flat disc of radius R: the launch point (x, y) is uniform on the disc, so r^2/R^2 is
+  uniform on (0, 1] and the polar angle uniform on (0, 2 pi];
+* TEM00 beam of 1/e^2 radius w: x and y are independent N(0, w/2), as the reference draws
+  them (pymopt/voxel/_classes.py:226-233);
+* exit tallies: every histogram the oracle accumulates on line is rebuilt here with numpy
+  from its own exit-state dump, following the reference's definitions:
+  Rd(r)/Tt(r) (pymopt/metrics/_scoring.py:16-25), top-face Rd(r) (pymopt/virtualOBD.py:158-166),
+  angular Rd/Tt (pymopt/metrics/_scoring.py:27-38), the side line detector
+  (pymopt/virtualOBD.py:171-172) and annular rings on the top face.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from pymonteopt_amd import geometry
+
+
+def _slab(thickness=1.0, pitch=0.1, size=6.1, ma=0.1, ms=5.0, g=0.8, n=1.4):
+    vol = geometry.plate_volume([thickness], pitch, size, size)
+    ctr = (geometry.center_add(vol.shape[0]), geometry.center_add(vol.shape[1]), 1)
+    return vol, pitch, [ma], [ms], [g], [n, 1.0], ctr
+
+
+def _launch_points(oracle, kind, radius, count, seed=3):
+    vol, l, ma, ms, g, n, ctr = _slab()
+    src = dict(kind=kind, voxel=ctr, radius=radius, dir=(0.0, 0.0, 1.0), w0=1.0)
+    cx, cy = ctr[0], ctr[1]
+    xs, ys = [], []
+    for pid in range(count):
+        ev = oracle.trace(vol, l, ma, ms, g, n, src, pid, seed, cap=1)
+        e = ev[0]
+        assert e["type"] == 0
+        a, p = e["a"], e["p"]
+        assert all(-l / 2 <= p[i] <= l / 2 for i in range(3))
+        assert a[2] == 1 and p[2] == pytest.approx(-l / 2)
+        xs.append(l * (a[0] - cx) + p[0])
+        ys.append(l * (a[1] - cy) + p[1])
+    return np.array(xs), np.array(ys)
+
+
+def test_flat_disc_launch_is_uniform_on_the_disc(oracle):
+    R = 1.0
+    x, y = _launch_points(oracle, 1, R, 3000)
+    r2 = (x * x + y * y) / R ** 2
+    assert r2.max() <= 1.0 + 1e-9
+    assert stats.kstest(r2, "uniform").pvalue > 1e-3
+    phi = np.mod(np.arctan2(y, x), 2 * np.pi) / (2 * np.pi)
+    assert stats.kstest(phi, "uniform").pvalue > 1e-3
+    # a radius drawn as R*u (not R*sqrt(u)) crowds the centre: E[r^2] would be R^2/3
+    assert abs(r2.mean() - 0.5) < 4 * math.sqrt(1 / 12 / r2.size)
+
+
+def test_tem00_launch_is_gaussian_with_sigma_half_w(oracle):
+    w = 1.0
+    x, y = _launch_points(oracle, 2, w, 3000)
+    for c in (x, y):
+        assert stats.kstest(c, "norm", args=(0.0, w / 2)).pvalue > 1e-3
+    assert abs(np.corrcoef(x, y)[0, 1]) < 4 / math.sqrt(x.size)
+
+
+def _binned(values, weights, width, nbins, select):
+    """The reference's (i*width, (i+1)*width] bins."""
+    out = np.zeros(nbins)
+    b = np.ceil(values / width).astype(np.int64) - 1
+    ok = select & (b >= 0) & (b < nbins)
+    np.add.at(out, b[ok], weights[ok])
+    return out
+
+
+def _edge_mass(values, weights, width, nbins, select, eps):
+    """Per bin, the weight of selected exits within eps of a bin edge, in that bin or a
+    neighbour: the float32 dump cannot say on which side of an edge the oracle's double
+    value fell."""
+    q = values / width
+    near = select & (np.abs(q - np.round(q)) * width < eps)
+    acc = np.zeros(nbins + 2)
+    b = np.clip(np.ceil(q).astype(np.int64) - 1, -1, nbins)
+    np.add.at(acc, b[near] + 1, weights[near])
+    return acc[:-2] + acc[1:-1] + acc[2:]
+
+
+def test_exit_tallies_match_their_definitions(oracle):
+    vol, l, ma, ms, g, n, ctr = _slab(thickness=2.0)
+    src = dict(kind=0, voxel=ctr, dir=(0.0, 0.0, 1.0), w0=0.97)
+    nr, dr, na, nl, dl, line_y = 30, 0.1, 10, 20, 0.1, 0.5
+    rings = [(0.2, 0.4), (0.5, 1.0), (1.25, 2.5)]
+    N = 20000
+    out = oracle.walk(vol, l, ma, ms, g, n, src, N, 11, nr=nr, dr=dr, na=na, nl=nl, dl=dl, line_y=line_y,
+                      rings=rings, absorbed=False, exit_state=True)
+    x, y, z, vx, vy, vz, w = (c.astype(np.float64) for c in out["exit_state"])
+    up, down = vz > 0, vz < 0
+    rel = 1e-6     # the dump holds float32 weights; the tallies sum the double ones
+
+    s = out["scalars"]
+    assert s["Rd"] == pytest.approx(w[down].sum(), rel=rel)
+    assert s["Tt"] == pytest.approx(w[up].sum(), rel=rel)
+
+    r = np.hypot(x, y)
+    top = down & (z < 1e-8)
+    for key, sel in (("rd_r", down), ("tt_r", up), ("rd_top_r", top)):
+        want = _binned(r, w, dr, nr, sel)
+        slack = _edge_mass(r, w, dr, nr, sel, 1e-5)
+        assert np.all(np.abs(out[key] - want) <= slack + rel * want + 1e-12), key
+    assert out["rd_top_r"].sum() > 0.5 * out["rd_r"].sum()
+
+    ang = np.arccos(np.abs(vz))
+    width = math.pi / (2 * na)
+    for key, sel in (("rd_a", down), ("tt_a", up)):
+        want = _binned(ang, w, width, na, sel)
+        slack = _edge_mass(ang, w, width, na, sel, 1e-3)
+        assert np.all(np.abs(out[key] - want) <= slack + rel * want + 1e-12), key
+
+    side = (vx < 0) & (np.abs(y) < line_y)
+    want = _binned(z, w, dl, nl, side)
+    slack = _edge_mass(z, w, dl, nl, side, 1e-5)
+    assert np.all(np.abs(out["line_z"] - want) <= slack + rel * want + 1e-12)
+    assert out["line_z"].sum() > 0
+
+    for i, (a, b) in enumerate(rings):
+        sel = top & (r > a) & (r <= b)
+        near = top & ((np.abs(r - a) < 1e-5) | (np.abs(r - b) < 1e-5))
+        assert abs(out["ring_w"][i] - w[sel].sum()) <= w[near].sum() + rel * w[sel].sum() + 1e-12
+        assert out["ring_w"][i] > 0
