@@ -11,6 +11,7 @@ import argparse
 import json
 import math
 import os
+import statistics
 import sys
 import time
 
@@ -145,7 +146,8 @@ def measure_peaks():
     for _ in range(2):
         e0, e1 = _events()
         e0.record()
-        _abi.check(L.pmo_probe_atomics(C.c_void_p(buf.data_ptr()), buf.numel(), blocks, iters, s))
+        _abi.check(L.pmo_probe_atomics(C.c_void_p(buf.data_ptr()), buf.numel(), blocks, iters,
+                                        _abi.PMO_PROBE_F64, s))
         e1.record()
         e1.synchronize()
         out["fp64_atomics_per_s"] = blocks * 256 * iters / (e0.elapsed_time(e1) * 1e-3)
@@ -275,6 +277,9 @@ def main():
             "tallies": "Rd, Tt, Rd(r) 40x0.1mm, absorbed/fluence map (FP64)" if spec.absorbed
             else "Rd, Tt, Rd(r) 40x0.1mm (FP64)",
             "walk_kernel_ms_mean": 1e3 * mean_launch_s,
+            "walk_kernel_ms_median": 1e3 * statistics.median(kernel_s),
+            "photons_per_s_per_step_median": N / statistics.median(kernel_s),
+            "photons_per_s_per_step_ci95": _ci95([N / t for t in kernel_s]),
             "events_per_packet": {k: ev_sum[k] / (N * args.steps) for k in ("interactions", "crossings", "fresnel",
                                                                               "roulette")},
             "Rd": [float(sum(rd) / len(rd)), float(_se(rd))],
@@ -285,22 +290,23 @@ def main():
         "roofline": {
             "bound": "valu",
             "achieved": achieved / 1e12,
-            "peak": roofline.PEAK_VALU_LANE_OPS / 1e12,
+            "peak": roofline.PEAK_VALU_PUBLISHED_LANE_OPS / 1e12,
             "unit": "Tlane-op/s",
-            "frac": achieved / roofline.PEAK_VALU_LANE_OPS,
+            "frac": achieved / roofline.PEAK_VALU_PUBLISHED_LANE_OPS,
             "traffic": None,
+            "peak_source": "MI355X published FP32 peak 157.3 TFLOP/s = 78.65e12 FMA lane-op slots/s",
             "ops_per_packet": ops,
             "hbm_algorithmic_GBps": N * bytes_pp / mean_launch_s / 1e9,
+            "hbm_algorithmic_frac": N * bytes_pp / mean_launch_s / roofline.PEAK_HBM_BPS,
         },
     }
     peaks = measure_peaks()
     glob_pp = ev_sum.get("global_deposits", 0) / (N * args.steps)
-    # VALU peak re-measured by libpmo's FMA probe (non-packed FP32); the nominal figure
-    # 256 CU x 64 lanes x 2.4 GHz is kept beside it
-    out["roofline"]["peak_nominal"] = out["roofline"]["peak"]
-    out["roofline"]["peak"] = peaks["valu_lane_ops_per_s"] / 1e12
-    out["roofline"]["frac"] = achieved / peaks["valu_lane_ops_per_s"]
-    out["roofline"]["frac_of_nominal_peak"] = achieved / roofline.PEAK_VALU_LANE_OPS
+    # the FP32 FMA issue rate re-measured by libpmo's probe (non-packed FP32, as the walk
+    # issues it) and the one-op-per-lane-per-clock figure are kept beside the published peak
+    out["roofline"]["peak_measured"] = peaks["valu_lane_ops_per_s"] / 1e12
+    out["roofline"]["frac_of_measured_peak"] = achieved / peaks["valu_lane_ops_per_s"]
+    out["roofline"]["peak_64_lanes_per_clock"] = roofline.PEAK_VALU_LANE_OPS / 1e12
     out["roofline_atomic"] = {
         "bound": "fp64_atomic",
         "achieved": glob_pp * N / mean_launch_s,
@@ -320,6 +326,17 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl)
     print(json.dumps(out), flush=True)
+
+
+def _ci95(x):
+    """Student-t 95% interval of the mean of x."""
+    n = len(x)
+    if n < 2:
+        return None
+    from scipy import stats
+    m = sum(x) / n
+    h = stats.t.ppf(0.975, n - 1) * _se(x)
+    return [m - h, m + h]
 
 
 def _se(x):

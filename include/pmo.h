@@ -224,10 +224,19 @@ int pmo_walk_adjoint(const pmo_volume* vol, const pmo_source* src, const pmo_run
 
 /* Diagnostics that re-measure the two peaks the walk is judged against (no reference
  * counterpart). The caller times them on `stream` (e.g. with HIP events):
- * pmo_probe_atomics issues blocks*256*iters FP64 atomic adds (no return) to pseudo-random
- * addresses of the device array buf[n]; pmo_probe_valu runs blocks*256 lanes that each
- * issue iters x 32 independent FP32 FMAs and write one float to out[blocks*256]. */
-int pmo_probe_atomics(double* buf, int64_t n, int32_t blocks, int32_t iters, void* stream);
+ * pmo_probe_atomics issues blocks*256*iters device-scope atomic adds (no return) of 1 to
+ * pseudo-random addresses of the device array buf[n], whose element type `kind` names; with
+ * PMO_PROBE_F64_PER_XCD, buf holds 8 copies of n doubles and each workgroup adds into the
+ * copy of the XCD it runs on. pmo_probe_valu runs blocks*256 lanes that each issue
+ * iters x 32 independent FP32 FMAs and write one float to out[blocks*256]. */
+enum {
+    PMO_PROBE_F64 = 0,
+    PMO_PROBE_F32 = 1,
+    PMO_PROBE_U64 = 2,
+    PMO_PROBE_U32 = 3,
+    PMO_PROBE_F64_PER_XCD = 4
+};
+int pmo_probe_atomics(void* buf, int64_t n, int32_t blocks, int32_t iters, int32_t kind, void* stream);
 int pmo_probe_valu(float* out, int32_t blocks, int32_t iters, void* stream);
 
 /* Host helper: the round-robin batch schedule of a sweep. Writes up to `cap` batches

@@ -24,6 +24,8 @@ PMO_BEAM_GAUSS = 2
 PMO_F_FP64 = 1 << 0
 PMO_F_NORMAL_FRESNEL = 1 << 1
 
+PMO_PROBE_F64, PMO_PROBE_F32, PMO_PROBE_U64, PMO_PROBE_U32, PMO_PROBE_F64_PER_XCD = 0, 1, 2, 3, 4
+
 SCALAR_NAMES = ["Rd", "Tt", "other", "A", "kill", "gain", "w0",
                 "face_x-", "face_x+", "face_y-", "face_y+", "face_z-", "face_z+"]
 PMO_S_COUNT = 13
@@ -91,7 +93,7 @@ SIGNATURES = {
                                C.c_void_p, C.POINTER(PmoDiff), C.c_void_p, C.c_void_p], C.c_int),
     "pmo_walk_adjoint": ([C.POINTER(PmoVolume), C.POINTER(PmoSource), C.POINTER(PmoRun), C.POINTER(PmoDiff),
                           C.c_void_p, C.c_void_p], C.c_int),
-    "pmo_probe_atomics": ([C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p], C.c_int),
+    "pmo_probe_atomics": ([C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_void_p], C.c_int),
     "pmo_probe_valu": ([C.c_void_p, C.c_int32, C.c_int32, C.c_void_p], C.c_int),
     "pmo_plate_schedule": ([C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int64], C.c_int64),
     "pmo_walk_plates": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.POINTER(PmoRun),

@@ -264,12 +264,32 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     return PMO_OK;
 }
 
-int pmo_probe_atomics(double* buf, int64_t n, int32_t blocks, int32_t iters, void* stream)
+int pmo_probe_atomics(void* buf, int64_t n, int32_t blocks, int32_t iters, int32_t kind, void* stream)
 {
     g_err[0] = '\0';
     if (!buf || n < 1 || n >= (1ll << 32) || blocks < 1 || iters < 1) return fail(PMO_ERR_INVALID, "bad probe size");
-    hipLaunchKernelGGL(pmo::probe_atomics_kernel, dim3(blocks), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
-                       buf, (uint32_t)n, iters);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const uint32_t m = (uint32_t)n;
+    switch (kind) {
+    case PMO_PROBE_F64:
+        hipLaunchKernelGGL(pmo::probe_atomics_kernel<double>, dim3(blocks), dim3(256), 0, s, (double*)buf, m, iters, 0);
+        break;
+    case PMO_PROBE_F32:
+        hipLaunchKernelGGL(pmo::probe_atomics_kernel<float>, dim3(blocks), dim3(256), 0, s, (float*)buf, m, iters, 0);
+        break;
+    case PMO_PROBE_U64:
+        hipLaunchKernelGGL(pmo::probe_atomics_kernel<unsigned long long>, dim3(blocks), dim3(256), 0, s,
+                           (unsigned long long*)buf, m, iters, 0);
+        break;
+    case PMO_PROBE_U32:
+        hipLaunchKernelGGL(pmo::probe_atomics_kernel<uint32_t>, dim3(blocks), dim3(256), 0, s, (uint32_t*)buf, m, iters, 0);
+        break;
+    case PMO_PROBE_F64_PER_XCD:
+        hipLaunchKernelGGL(pmo::probe_atomics_kernel<double>, dim3(blocks), dim3(256), 0, s, (double*)buf, m, iters, 1);
+        break;
+    default:
+        return fail(PMO_ERR_INVALID, "unknown probe kind");
+    }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PMO_OK : hip_fail(e, "probe launch");
 }

@@ -624,12 +624,19 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n)
 }
 
 // probes of the FP64 atomic rate and the FP32 FMA issue rate (every index in bounds)
-__global__ void probe_atomics_kernel(double* buf, uint32_t n, int iters)
+// XCD (accelerator complex die) the calling wave runs on: HW_REG_XCC_ID (hwreg 20), bits 3:0
+__device__ __forceinline__ uint32_t xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 7u; }
+
+// Device-scope atomic adds (no return) of T to pseudo-random addresses of buf[n]; with
+// per_xcd, buf holds 8 copies of n elements and each workgroup adds into its XCD's copy.
+template <class T>
+__global__ void probe_atomics_kernel(T* buf, uint32_t n, int iters, int per_xcd)
 {
-    uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t x = (blockIdx.x * blockDim.x + threadIdx.x) * 2654435761u ^ 2463534242u;
+    T* base = per_xcd ? buf + (size_t)xcc_id() * n : buf;
     for (int i = 0; i < iters; ++i) {
         x ^= x << 13; x ^= x >> 17; x ^= x << 5;          // xorshift32
-        gmem_add(buf + (x % n), 1.0);
+        __hip_atomic_fetch_add(base + (x % n), (T)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 

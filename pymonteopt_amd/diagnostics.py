@@ -1,7 +1,8 @@
 """Measurement helpers run from the command line.
 
     python -m pymonteopt_amd.diagnostics copy     # 3 copies of 256 MiB (counter calibration)
-    python -m pymonteopt_amd.diagnostics peaks    # FP64 atomic rate and FP32 FMA issue rate
+    python -m pymonteopt_amd.diagnostics peaks    # atomic-add rates (f64/f32/u64/u32, per-XCD
+                                                  # copies) and the FP32 FMA issue rate
 
 `copy` gives rocprofv3's FETCH_SIZE / WRITE_SIZE a kernel of known traffic (read 256 MiB,
 write 256 MiB per dispatch), so walk-kernel counts can be scaled to bytes.
@@ -40,10 +41,29 @@ def peaks():
         return e0.elapsed_time(e1) * 1e-3
 
     out = {}
+    blocks, iters = 4096, 256
+    count = blocks * 256 * iters
+    kinds = (("f64", _abi.PMO_PROBE_F64, 8), ("f32", _abi.PMO_PROBE_F32, 4), ("u64", _abi.PMO_PROBE_U64, 8),
+             ("u32", _abi.PMO_PROBE_U32, 4))
     for mib in (0.25, 2, 32, 256):
-        buf = torch.zeros(int(mib * 1024 * 1024 // 8), dtype=torch.float64, device="cuda:0")
-        t = timed(lambda: _abi.check(L.pmo_probe_atomics(C.c_void_p(buf.data_ptr()), buf.numel(), 4096, 256, s)))
-        out[f"fp64_atomics_per_s_{mib}MiB"] = 4096 * 256 * 256 / t
+        for name, kind, size in kinds:
+            if name != "f64" and mib not in (2, 32):
+                continue
+            buf = torch.zeros(int(mib * 1024 * 1024 // size), dtype={8: torch.float64, 4: torch.float32}[size],
+                              device="cuda:0")
+            t = timed(lambda: _abi.check(L.pmo_probe_atomics(C.c_void_p(buf.data_ptr()), buf.numel(), blocks, iters,
+                                                             kind, s)))
+            out[f"{name}_atomics_per_s_{mib}MiB"] = count / t
+        if mib in (2, 32):
+            # 8 copies of `mib` MiB, each workgroup adding into the copy of its own XCD
+            n = int(mib * 1024 * 1024 // 8)
+            buf = torch.zeros(8 * n, dtype=torch.float64, device="cuda:0")
+            t = timed(lambda: _abi.check(L.pmo_probe_atomics(C.c_void_p(buf.data_ptr()), n, blocks, iters,
+                                                             _abi.PMO_PROBE_F64_PER_XCD, s)))
+            out[f"f64_per_xcd_copy_atomics_per_s_{mib}MiB"] = count / t
+            torch.cuda.synchronize()
+            # every add of 1.0 must have landed: a lost update would show as a short sum
+            out[f"f64_per_xcd_copy_sum_ok_{mib}MiB"] = bool(float(buf.sum()) == 2.0 * count)
     o = torch.empty(2048 * 256, dtype=torch.float32, device="cuda:0")
     t = timed(lambda: _abi.check(L.pmo_probe_valu(C.c_void_p(o.data_ptr()), 2048, 8192, s)))
     out["valu_fp32_fma_lane_ops_per_s"] = 2048 * 256.0 * 8192 * 32 / t

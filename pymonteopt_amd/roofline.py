@@ -22,7 +22,11 @@ WEIGHTS = {
 N_CU = 256
 LANES_PER_CU_CLK = 64
 CLOCK_HZ = 2.4e9
-PEAK_VALU_LANE_OPS = N_CU * LANES_PER_CU_CLK * CLOCK_HZ     # 3.93e13 lane-op/s
+PEAK_VALU_LANE_OPS = N_CU * LANES_PER_CU_CLK * CLOCK_HZ     # 3.93e13 lane-op/s, one op per lane per clock
+# AMD's published MI355X FP32 vector peak, 157.3 TFLOP/s, counts an FMA as 2 flops and
+# assumes packed (two-wide) FP32; in the lane-op slots of WEIGHTS it is 7.86e13 slots/s
+PEAK_FP32_PUBLISHED_FLOPS = 157.3e12
+PEAK_VALU_PUBLISHED_LANE_OPS = PEAK_FP32_PUBLISHED_FLOPS / 2
 PEAK_HBM_BPS = 8.0e12
 
 

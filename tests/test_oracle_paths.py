@@ -9,7 +9,8 @@ Monte Carlo statistics the adjoint is checked against) on the CPU.
   R^{2k-1} (1-R) e^{-2k mu_a d}; its interaction count is zero;
 * path sums, scattering 3-layer slab: sum_j of w L_j and w n_j recounted packet by packet
   from the oracle's own event trace (segment lengths between consecutive events, in the
-  label of the voxel the segment lies in; interactions per label).
+  label of the voxel the segment lies in; interactions per label), and the walk's event
+  counters recounted from the same trace.
 """
 import math
 
@@ -79,7 +80,7 @@ def test_path_sums_non_scattering_closed_form(oracle):
 def test_path_sums_match_the_event_trace(oracle):
     l = 0.05
     vol = plate_volume([0.1, 0.3, 0.6], l, 2.05, 2.05)
-    opt = optics_table([0.5, 0.2, 0.05], [20.0, 10.0, 5.0], [0.8, 0.9, 0.7], [1.44, 1.38, 1.5], 1.0)
+    opt = optics_table([0.5, 0.2, 1.5], [20.0, 10.0, 5.0], [0.8, 0.9, 0.7], [1.44, 1.38, 1.5], 1.0)
     cols = _cols(opt)
     cx, cy = center_add(vol.shape[0]), center_add(vol.shape[1])
     src = dict(kind=0, voxel=(cx, cy, 1), w0=1 - specular(1.0, 1.44))
@@ -106,6 +107,16 @@ def test_path_sums_match_the_event_trace(oracle):
         r = math.hypot(x, y)
         binned = ev[-1]["type"] == 4 and vz < 0 and 0 < r <= nr * dr
         wl, wn = out["rd_wl"].sum(axis=0), out["rd_wn"].sum(axis=0)
+        # event counters: a kill ends an interaction before its (post-spin) trace event
+        types = [e["type"] for e in ev]
+        nidx = lambda a: float(opt[int(vol[a]), 3])
+        fres = sum(1 for a, b in zip(ev[:-1], ev[1:])
+                   if b["type"] == 2 or (b["type"] in (1, 4) and nidx(a["a"]) != nidx(b["a"])))
+        gains = sum(1 for a, b in zip(ev[:-1], ev[1:]) if b["type"] == 3 and b["w"] > a["w"])
+        assert out["events"] == dict(packets=1, interactions=types.count(3) + types.count(5),
+                                     crossings=types.count(1) + types.count(2) + types.count(4),
+                                     fresnel=fres, roulette=gains + types.count(5), killed=types.count(5),
+                                     escaped=types.count(4))
         if binned:
             checked += 1
             assert wl == pytest.approx(w * Lj, rel=1e-9, abs=1e-12)
