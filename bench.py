@@ -54,7 +54,8 @@ def _events():
 def batched_leg(n_inst=512):
     """512 random slabs (uneven budgets) in one graph-captured persistent launch, against
     512 sequential plain walks of the same plates with the same draws (so each instance's
-    estimate, and its standard error, is the same in both)."""
+    estimate, and its standard error, is the same in both). Reported for layer stepping
+    (the sweep's default) and for voxel stepping, which reproduces the plain walks exactly."""
     import numpy as np
     import torch
     from pymonteopt_amd import autotune
@@ -63,18 +64,25 @@ def batched_leg(n_inst=512):
     from pymonteopt_amd.runtime import TallySpec, Walker
     plates = slab_sweep(n_inst, seed=0)
     N = sum(p.n_packets for p in plates)
+
+    def graph_time(sw):
+        sw.capture()
+        sw.replay(1)
+        torch.cuda.synchronize()
+        e0, e1 = _events()
+        e0.record()
+        sw.replay(2)
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) * 1e-3, sw.results()
+
+    # layer stepping (the default) and voxel stepping, each autotuned, graph-captured
     sw = PlateSweep(plates, nr=40, dr=0.1, batch=256)
-    tuned = autotune.tune_sweep(sw, blocks=(128, 256), batches=(128, 256, 512))
-    sw.capture()
-    sw.replay(1)
-    torch.cuda.synchronize()
-    e0, e1 = _events()
-    e0.record()
-    sw.replay(2)
-    e1.record()
-    e1.synchronize()
-    tb = e0.elapsed_time(e1) * 1e-3
-    rb = sw.results()
+    tuned = autotune.tune_sweep(sw, blocks=(128, 256), batches=(128, 256, 512), quorums=(32, 24, 48))
+    tb, rb = graph_time(sw)
+    swv = PlateSweep(plates, nr=40, dr=0.1, batch=256, layer_step=False)
+    tuned_v = autotune.tune_sweep(swv, blocks=(128, 256), batches=(128, 256, 512), quorums=(32, 24, 48))
+    tv, rv = graph_time(swv)
     walkers = []
     for p in plates:
         kind, vox, w0 = p.source()
@@ -82,6 +90,7 @@ def batched_leg(n_inst=512):
     spec = TallySpec(nr=40, dr=0.1, absorbed=False)
     tals = [w.tallies(spec) for w, _ in walkers]
     torch.cuda.synchronize()
+    e0, e1 = _events()
     e0.record()
     for i, ((w, s), t) in enumerate(zip(walkers, tals)):
         w.walk(s, plates[i].n_packets, 2, t, stream_id=i)
@@ -89,10 +98,14 @@ def batched_leg(n_inst=512):
     e1.synchronize()
     tp = e0.elapsed_time(e1) * 1e-3
     rd_p = np.array([t.to_host()["scalars"]["Rd"] / p.n_packets for t, p in zip(tals, plates)])
-    return {"instances": n_inst, "packets": N, "batched_s": tb, "batched_photons_per_s": N / tb,
+    strip = lambda r: {k: v for k, v in r.items() if k != "table"}
+    return {"instances": n_inst, "packets": N, "stepping": "layer", "batched_s": tb, "batched_photons_per_s": N / tb,
             "sequential_plain_s": tp, "sequential_photons_per_s": N / tp, "speedup": tp / tb,
-            "max_abs_Rd_diff_same_draws": float(np.max(np.abs(rd_p - rb["Rd"]))),
-            "tuned": {k: v for k, v in tuned.items() if k != "table"}, "graph": True}
+            "max_abs_Rd_diff_same_draws": float(np.max(np.abs(rd_p - rb["Rd"]))), "tuned": strip(tuned),
+            "voxel_stepping": {"batched_s": tv, "batched_photons_per_s": N / tv, "speedup": tp / tv,
+                               "max_abs_Rd_diff_same_draws": float(np.max(np.abs(rd_p - rv["Rd"]))),
+                               "tuned": strip(tuned_v)},
+            "graph": True}
 
 
 def adjoint_leg(n_packets=10_000_000):

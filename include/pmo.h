@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define PMO_ABI_VERSION 1
+#define PMO_ABI_VERSION 2
 
 /* ---- error codes ---------------------------------------------------------------- */
 #define PMO_OK 0
@@ -55,6 +55,14 @@ extern "C" {
 #define PMO_F_NORMAL_FRESNEL (1u << 1) /* at exactly normal incidence use the Fresnel limit
                                           ((ni-nt)/(ni+nt))^2; default keeps R = 0 there,
                                           as pymopt/voxel/_kernel.py:89-93 does            */
+#define PMO_F_LAYER_STEP (1u << 2)     /* pmo_walk_plates only: step from layer boundary to
+                                          layer boundary and lateral wall in continuous
+                                          coordinates instead of from voxel face to voxel
+                                          face. In a plate the skipped faces change neither
+                                          label nor index, so the walk is the same in exact
+                                          arithmetic and uses the same draws; results equal
+                                          the voxel walk's up to FP32 rounding. Needs every
+                                          instance's layer table.                           */
 
 /* ---- scalar tally slots (double) ---------------------------------------------------- */
 enum {
@@ -157,6 +165,12 @@ typedef struct pmo_plate_instance {
     int32_t src_kind, ix, iy, iz; /* launch (pmo_source fields)                                 */
     float radius, dir[3], w0;
     uint64_t n_packets;      /* this instance's budget (< 2^32)                                    */
+    const float* layers;     /* device float4 [n_layers]: (z_top, z_bottom, label, 0) of each run of
+                                equal labels in z_labels[1 .. nz-2], top to bottom, z in mm below the
+                                top face (slice i spans pitch*(i-1) .. pitch*i); read with
+                                PMO_F_LAYER_STEP, else may be NULL                                  */
+    int32_t n_layers;
+    int32_t reserved;        /* 0                                                                  */
 } pmo_plate_instance;
 
 typedef struct pmo_batch_tallies {

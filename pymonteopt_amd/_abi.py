@@ -11,6 +11,8 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpmo.so")
 
+ABI_VERSION = 2          # PMO_ABI_VERSION of include/pmo.h
+
 PMO_OK = 0
 PMO_ERR_INVALID = -1
 PMO_ERR_LABEL = -2
@@ -23,6 +25,7 @@ PMO_BEAM_GAUSS = 2
 
 PMO_F_FP64 = 1 << 0
 PMO_F_NORMAL_FRESNEL = 1 << 1
+PMO_F_LAYER_STEP = 1 << 2
 
 PMO_PROBE_F64, PMO_PROBE_F32, PMO_PROBE_U64, PMO_PROBE_U32, PMO_PROBE_F64_PER_XCD = 0, 1, 2, 3, 4
 
@@ -66,7 +69,8 @@ class PmoPlateInstance(C.Structure):
     _fields_ = [("z_labels", C.c_void_p), ("optics", C.c_void_p), ("nx", C.c_int32), ("ny", C.c_int32),
                 ("nz", C.c_int32), ("n_labels", C.c_int32), ("pitch", C.c_float), ("src_kind", C.c_int32),
                 ("ix", C.c_int32), ("iy", C.c_int32), ("iz", C.c_int32), ("radius", C.c_float),
-                ("dir", C.c_float * 3), ("w0", C.c_float), ("n_packets", C.c_uint64)]
+                ("dir", C.c_float * 3), ("w0", C.c_float), ("n_packets", C.c_uint64), ("layers", C.c_void_p),
+                ("n_layers", C.c_int32), ("reserved", C.c_int32)]
 
 
 class PmoBatchTallies(C.Structure):
@@ -122,7 +126,7 @@ def lib():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = res
-        if L.pmo_abi_version() != 1:
+        if L.pmo_abi_version() != ABI_VERSION:
             raise ImportError("libpmo ABI version mismatch")
         _lib = L
     return _lib

@@ -81,31 +81,44 @@ def tune_walk(walker, source, spec, n_packets, *, sample=2_000_000, blocks=(128,
     return res
 
 
-def tune_sweep(sweep, *, blocks=(128, 256, 512, 1024), batches=(64, 128, 256, 512, 1024), seed=0,
+def tune_sweep(sweep, *, blocks=(128, 256, 512, 1024), batches=(64, 128, 256, 512, 1024), quorums=(32,), seed=0,
                cache_path=None):
-    """Best (block_size, batch) for a PlateSweep; leaves the sweep set to the winner."""
+    """Best (block_size, batch, quorum) for a PlateSweep, searched one after the other
+    (block and batch together, then the quorum at the best pair); leaves the sweep set to
+    the winner."""
     dev = torch.cuda.get_device_name(sweep.device)
     sig = tuple((tuple(p.thickness), tuple(p.ma), tuple(p.ms), p.n_packets) for p in sweep.plates[:8])
-    key = f"sweep|{dev}|{sweep.n_inst}|{sweep.total_packets}|{hash(sig)}"
+    key = f"sweep|{dev}|{sweep.n_inst}|{sweep.total_packets}|{sweep.layer_step}|{hash(sig)}"
     if key in _CACHE:
         res = _CACHE[key]
     else:
         res = _load(cache_path).get(key)
     if res is None:
         table = []
-        orig = sweep.batch
+        orig_batch, orig_q = sweep.batch, sweep.quorum
+        q0 = quorums[0]
+        sweep.quorum = q0
         for b in batches:
             sweep.set_batch(b)
             for bs in blocks:
                 t = _time(lambda: sweep.run(seed, block_size=bs), reps=1)
-                table.append({"block_size": bs, "batch": b, "seconds": t,
+                table.append({"block_size": bs, "batch": b, "quorum": q0, "seconds": t,
                               "photons_per_s": sweep.total_packets / t})
-        sweep.set_batch(orig)
         best = min(table, key=lambda r: r["seconds"])
-        res = {"block_size": best["block_size"], "batch": best["batch"], "photons_per_s": best["photons_per_s"],
-               "table": table}
+        sweep.set_batch(best["batch"])
+        for q in quorums[1:]:
+            sweep.quorum = q
+            t = _time(lambda: sweep.run(seed, block_size=best["block_size"]), reps=1)
+            table.append({"block_size": best["block_size"], "batch": best["batch"], "quorum": q, "seconds": t,
+                          "photons_per_s": sweep.total_packets / t})
+        best = min(table, key=lambda r: r["seconds"])
+        sweep.set_batch(orig_batch)
+        sweep.quorum = orig_q
+        res = {"block_size": best["block_size"], "batch": best["batch"], "quorum": best["quorum"],
+               "photons_per_s": best["photons_per_s"], "table": table}
         _CACHE[key] = res
         _store(cache_path, key, res)
     sweep.set_batch(res["batch"])
     sweep.block_size = res["block_size"]
+    sweep.quorum = res.get("quorum", 0)
     return res

@@ -56,6 +56,20 @@ class Plate:
             z += k
         return col
 
+    def layer_table(self):
+        """(z_top, z_bottom, label, 0) of each run of equal labels in the interior slices,
+        top to bottom, in mm below the top face (slice i spans (i-1)*l .. i*l)."""
+        col = self.z_labels()
+        l = self.voxel_space
+        rows = []
+        z, nz = 1, col.size
+        while z < nz - 1:
+            z0, lab = z, int(col[z])
+            while z < nz - 1 and int(col[z]) == lab:
+                z += 1
+            rows.append((l * (z0 - 1), l * (z - 1), float(lab), 0.0))
+        return np.array(rows, np.float32).reshape(-1, 4)
+
     def optics(self):
         return optics_table(self.ma, self.ms, self.g, self.n, self.n_air)
 
@@ -91,8 +105,12 @@ def slab_sweep(n_inst=512, seed=0, pitch=0.05, lateral=101, budget=(1e5, 1e6), t
 class PlateSweep:
     """Device-resident instances, schedule and per-instance FP64 tallies of a sweep."""
 
-    def __init__(self, plates, nr=40, dr=0.1, batch=256, device="cuda:0"):
+    def __init__(self, plates, nr=40, dr=0.1, batch=256, device="cuda:0", layer_step=True):
+        """layer_step: walk each plate from layer boundary to layer boundary and lateral wall
+        (the default), or from voxel face to voxel face, which reproduces pmo_walk on the
+        plate's voxel volume to the last bit. Both use the same draws."""
         self.plates = list(plates)
+        self.layer_step = bool(layer_step)
         self.n_inst = len(self.plates)
         if self.n_inst < 1:
             raise ValueError("need at least one plate")
@@ -102,12 +120,15 @@ class PlateSweep:
         # packed z-label columns and optics tables
         cols = [p.z_labels() for p in self.plates]
         opts = [p.optics() for p in self.plates]
+        lays = [p.layer_table() for p in self.plates]
         zoff = np.cumsum([0] + [c.size for c in cols])
         ooff = np.cumsum([0] + [o.shape[0] for o in opts])
+        loff = np.cumsum([0] + [t.shape[0] for t in lays])
         self.z_labels = torch.from_numpy(np.concatenate(cols)).to(self.device)
         self.optics = torch.from_numpy(np.concatenate(opts).astype(np.float32)).to(self.device)
+        self.layers = torch.from_numpy(np.concatenate(lays).astype(np.float32)).to(self.device)
         desc = (_abi.PmoPlateInstance * self.n_inst)()
-        zb, ob = self.z_labels.data_ptr(), self.optics.data_ptr()
+        zb, ob, lb = self.z_labels.data_ptr(), self.optics.data_ptr(), self.layers.data_ptr()
         for i, p in enumerate(self.plates):
             d = desc[i]
             nx, ny, nz = p.shape()
@@ -123,6 +144,8 @@ class PlateSweep:
             d.dir[0], d.dir[1], d.dir[2] = 0.0, 0.0, 1.0
             d.w0 = w0
             d.n_packets = int(p.n_packets)
+            d.layers = lb + int(loff[i]) * 16
+            d.n_layers = int(lays[i].shape[0])
         raw = np.frombuffer(bytes(desc), dtype=np.uint8).copy()
         self.desc = torch.from_numpy(raw).to(self.device)
         self.set_batch(batch)
@@ -134,6 +157,7 @@ class PlateSweep:
         self.workspace = torch.zeros(int(self.lib.pmo_workspace_bytes()), dtype=torch.uint8, device=self.device)
         self.key = torch.zeros(2, dtype=torch.int32, device=self.device)
         self.block_size = 0          # 0 = library default; autotune.tune_sweep sets the best
+        self.quorum = 0              # lanes ready before a wave runs interactions; 0 = library default
         self._graph = None
 
     def set_batch(self, batch):
@@ -166,8 +190,9 @@ class PlateSweep:
         r.seed = int(seed) & 0xFFFFFFFF
         r.stream = int(stream_id) & 0xFFFFFFFF
         r.wth, r.roulette_m = float(wth), float(roulette_m)
-        r.flags = _abi.PMO_F_NORMAL_FRESNEL if normal_fresnel else 0
+        r.flags = (_abi.PMO_F_NORMAL_FRESNEL if normal_fresnel else 0) | (_abi.PMO_F_LAYER_STEP if self.layer_step else 0)
         r.block_size = int(block_size)
+        r.quorum = int(self.quorum)
         r.key_dev = C.c_void_p(self.key.data_ptr()) if key_dev else None
         t = _abi.PmoBatchTallies()
         t.scalars = C.c_void_p(self.scalars.data_ptr())

@@ -25,6 +25,8 @@ struct PlateInstance {     // device layout of pmo_plate_instance (include/pmo.h
     int32_t src_kind, ix, iy, iz;
     float radius, dirx, diry, dirz, w0;
     unsigned long long n_packets;
+    const float4* layers;       // (z_top, z_bottom, label, 0) per run of equal labels, top to bottom
+    int32_t n_layers, reserved;
 };
 
 struct BatchParams {
@@ -36,6 +38,7 @@ struct BatchParams {
     float wth, roulette_m, inv_m;
     uint32_t flags;
     uint32_t max_events;
+    uint32_t quorum;            // lanes wanting an interaction before a wave runs that step, >= 1
     double* scalars;            // [n_inst][kBS_COUNT]
     unsigned long long* events; // [kBE_COUNT]
     int32_t nr;
@@ -52,6 +55,8 @@ struct PlateGeo {
     const float4* optics;
     int32_t nx, ny, nz;
     uint32_t end;
+    const float4* layers;
+    int32_t n_layers;
 };
 
 __device__ __forceinline__ uint32_t plate_label(const PlateGeo& G, int32_t bx, int32_t by, int32_t bz)
@@ -70,7 +75,13 @@ __device__ __forceinline__ uint32_t plate_next_label(const PlateGeo& G, uint32_t
     return (bx < 1 || by < 1 || bx > G.nx - 2 || by > G.ny - 2) ? G.end : lab;
 }
 
-template <typename R>
+// LAYER = false: the voxel walk of pymopt/voxel/_kernel.py on the plate's voxels, identical
+// to pmo_walk on the same plate. LAYER = true: the same walk with the position kept in plate
+// coordinates (mm from the top-face centre) and faces only where the label can change: the
+// layer boundaries and the lateral walls. The voxel faces it skips change neither label nor
+// index (_kernel.py:82,110-118 do nothing there but st -= db*mt), so the two agree in exact
+// arithmetic and draw the same random numbers.
+template <typename R, bool LAYER>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5)))
 plate_batch_kernel(const BatchParams P)
 {
@@ -89,7 +100,8 @@ plate_batch_kernel(const BatchParams P)
     R px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0, ivx = 0, ivy = 0, ivz = 0;
     R w = 0, st = 0, u_roul = 0, u_th = 0, u_phi = 0, fb0 = 0, fb1 = 0, fb2 = 0, half = 0;
     LabelConst<R> L;                  // optics of the current label
-    PlateGeo G{nullptr, nullptr, 0, 0, 0, 0u};
+    PlateGeo G{nullptr, nullptr, 0, 0, 0, 0u, nullptr, 0};
+    R zt = 0, zb = 0, wxl = 0, wxh = 0, wyl = 0, wyh = 0;   // LAYER: current layer, lateral walls
     int32_t ax = 0, ay = 0, az = 0;
     uint32_t lab = 0, inter = 0, fres = 0, nev = 0, pid = 0, key1 = 0;
     int32_t inst = -1;
@@ -132,6 +144,7 @@ plate_batch_kernel(const BatchParams P)
                     const PlateInstance& I = P.inst[inst];
                     G.z_labels = I.z_labels; G.optics = I.optics;
                     G.nx = I.nx; G.ny = I.ny; G.nz = I.nz; G.end = (uint32_t)I.n_labels;
+                    G.layers = I.layers; G.n_layers = I.n_layers;
                     pid = b_next + rank;
                     key1 = skey + (uint32_t)inst;
                     half = (R)I.pitch * (R)0.5;
@@ -145,6 +158,26 @@ plate_batch_kernel(const BatchParams P)
                     w = (R)I.w0;
                     acc_w0 += (double)I.w0;
                     lab = plate_label(G, ax, ay, az);
+                    if constexpr (LAYER) {
+                        // plate coordinates of the launch point (the exit encoding of
+                        // voxel/_classes.py:261-269) and the walls of the interior
+                        const R pitch = (R)I.pitch;
+                        const int32_t cx = (I.nx - 1) / 2, cy = (I.ny - 1) / 2;
+                        px = pitch * (R)(ax - cx) + px;
+                        py = pitch * (R)(ay - cy) + py;
+                        pz = pitch * (R)(az - 1) + pz + half;
+                        wxl = pitch * (R)(1 - cx) - half; wxh = pitch * (R)(I.nx - 2 - cx) + half;
+                        wyl = pitch * (R)(1 - cy) - half; wyh = pitch * (R)(I.ny - 2 - cy) + half;
+                        // the layer holding the launch depth; none (end label) without a table
+                        az = -1;
+                        if (lab != G.end && G.layers != nullptr) {
+                            for (int32_t j = 0; j < G.n_layers; ++j) {
+                                const float4 Ly = G.layers[j];
+                                if (pz < (R)Ly.y) { az = j; zt = (R)Ly.x; zb = (R)Ly.y; lab = (uint32_t)Ly.z; break; }
+                            }
+                        }
+                        if (az < 0 || lab > G.end) lab = G.end;
+                    }
                     L = label_const<R>(G.optics[lab]);
                     inter = 0; fres = 0; nev = 0;
                     const uint4 r = philox4x32_10(make_uint4(0u, kTagWalk, pid, 0u), seed, key1);
@@ -156,37 +189,75 @@ plate_batch_kernel(const BatchParams P)
             }
         }
         if (__ballot(alive) == 0ull) break;
+
+        // classify every walking lane as a face crossing or an interaction, then run one
+        // kind per wave step: interactions once `quorum` lanes want one (or none crosses),
+        // so the long interaction block does not run with most of the wave masked off
+        const uint32_t end = G.end;
+        R db = (R)1000, smt = (R)0;
+        int k = 0;
+        bool want_int = false;
+        const bool walking = alive && lab != end;
+        if (walking) {
+            // only components with |v_i| > 0 take part (_kernel.py:65), as in the reference
+            R d0, d1, d2;
+            if constexpr (LAYER) {
+                d0 = vx != (R)0 ? (vx > (R)0 ? wxh - px : px - wxl) * ivx : (R)INFINITY;
+                d1 = vy != (R)0 ? (vy > (R)0 ? wyh - py : py - wyl) * ivy : (R)INFINITY;
+                d2 = vz != (R)0 ? (vz > (R)0 ? zb - pz : pz - zt) * ivz : (R)INFINITY;
+            } else {
+                d0 = vx != (R)0 ? (half - (vx > (R)0 ? px : -px)) * ivx : (R)INFINITY;
+                d1 = vy != (R)0 ? (half - (vy > (R)0 ? py : -py)) * ivy : (R)INFINITY;
+                d2 = vz != (R)0 ? (half - (vz > (R)0 ? pz : -pz)) * ivz : (R)INFINITY;
+            }
+            if (d0 < db) { db = d0; k = 0; }
+            if (d1 < db) { db = d1; k = 1; }
+            if (d2 < db) { db = d2; k = 2; }
+            smt = db * L.mt;
+            want_int = !(st >= smt);
+        }
+        const unsigned long long m_int = __ballot(walking && want_int);
+        const unsigned long long m_cross = __ballot(walking && !want_int);
+        const bool run_int = m_cross == 0ull || (uint32_t)__popcll(m_int) >= P.quorum;
         if (!alive) continue;
 
-        const uint32_t end = G.end;
         bool ended = false, killed = false;
         int xk = 2, xs = 1;
         if (lab == end) {
             ended = true; xs = vz > (R)0 ? 1 : -1;
-        } else {
+        } else if (want_int == run_int) {
             ++nev;
-            // only components with |v_i| > 0 take part (_kernel.py:65), as in the reference
-            const R d0 = vx != (R)0 ? (half - (vx > (R)0 ? px : -px)) * ivx : (R)INFINITY;
-            const R d1 = vy != (R)0 ? (half - (vy > (R)0 ? py : -py)) * ivy : (R)INFINITY;
-            const R d2 = vz != (R)0 ? (half - (vz > (R)0 ? pz : -pz)) * ivz : (R)INFINITY;
-            R db = (R)1000;
-            int k = 0;
-            if (d0 < db) { db = d0; k = 0; }
-            if (d1 < db) { db = d1; k = 1; }
-            if (d2 < db) { db = d2; k = 2; }
-            const R smt = db * L.mt;
-            if (st >= smt) {
+            if (!want_int) {
                 ++n_cross;
                 px += vx * db; py += vy * db; pz += vz * db;
                 const R vk = k == 0 ? vx : (k == 1 ? vy : vz);
-                const R face = M::copysign(half, vk);
-                if (k == 0) px = face; else if (k == 1) py = face; else pz = face;
-                st -= smt;
                 const int s = vk > (R)0 ? 1 : -1;
-                const int32_t bx = ax + (k == 0 ? s : 0);
-                const int32_t by = ay + (k == 1 ? s : 0);
-                const int32_t bz = az + (k == 2 ? s : 0);
-                const uint32_t nxt = plate_next_label(G, lab, k, bx, by, bz);
+                st -= smt;
+                uint32_t nxt = end;
+                int32_t bx = ax, by = ay, bz = az;
+                R nzt = zt, nzb = zb;
+                if constexpr (LAYER) {
+                    // onto the wall or layer boundary crossed; only z faces lead to a layer
+                    if (k == 0) px = s > 0 ? wxh : wxl;
+                    else if (k == 1) py = s > 0 ? wyh : wyl;
+                    else pz = s > 0 ? zb : zt;
+                    if (k == 2) {
+                        bz = az + s;
+                        if (bz >= 0 && bz < G.n_layers) {
+                            const float4 Ly = G.layers[bz];
+                            const uint32_t l = (uint32_t)Ly.z;
+                            nxt = l < end ? l : end;
+                            nzt = (R)Ly.x; nzb = (R)Ly.y;
+                        }
+                    }
+                } else {
+                    const R face = M::copysign(half, vk);
+                    if (k == 0) px = face; else if (k == 1) py = face; else pz = face;
+                    bx = ax + (k == 0 ? s : 0);
+                    by = ay + (k == 1 ? s : 0);
+                    bz = az + (k == 2 ? s : 0);
+                    nxt = plate_next_label(G, lab, k, bx, by, bz);
+                }
                 bool transmit = true;
                 LabelConst<R> Ln = L;
                 if (nxt != lab) {
@@ -211,8 +282,12 @@ plate_batch_kernel(const BatchParams P)
                     }
                 }
                 if (transmit) {
-                    ax = bx; ay = by; az = bz;
-                    if (k == 0) px = -px; else if (k == 1) py = -py; else pz = -pz;
+                    if constexpr (LAYER) {
+                        az = bz; zt = nzt; zb = nzb;
+                    } else {
+                        ax = bx; ay = by; az = bz;
+                        if (k == 0) px = -px; else if (k == 1) py = -py; else pz = -pz;
+                    }
                     if (nxt == end) {
                         ended = true; xk = k; xs = s;
                     } else if (nxt != lab) {
@@ -267,10 +342,15 @@ plate_batch_kernel(const BatchParams P)
                 gmem_add(sc + (fvz < 0.f ? kBS_RD : (fvz > 0.f ? kBS_TT : kBS_OTHER)), wd);
                 gmem_add(sc + kBS_FACE + 2 * xk + (xs > 0 ? 1 : 0), wd);
                 if (P.nr > 0 && fvz != 0.f) {
-                    const float pitch = P.inst[inst].pitch;
-                    const int cx = (G.nx - 1) / 2, cy = (G.ny - 1) / 2;
-                    const float x = pitch * (float)(ax - cx) + (float)px;
-                    const float y = pitch * (float)(ay - cy) + (float)py;
+                    float x, y;
+                    if constexpr (LAYER) {
+                        x = (float)px; y = (float)py;
+                    } else {
+                        const float pitch = P.inst[inst].pitch;
+                        const int cx = (G.nx - 1) / 2, cy = (G.ny - 1) / 2;
+                        x = pitch * (float)(ax - cx) + (float)px;
+                        y = pitch * (float)(ay - cy) + (float)py;
+                    }
                     const int b = bin_of(__builtin_sqrtf(x * x + y * y), P.dr, P.nr);
                     if (b >= 0) {
                         double* h = fvz < 0.f ? P.rd_r : P.tt_r;

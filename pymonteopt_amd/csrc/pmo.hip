@@ -515,6 +515,8 @@ int pmo_walk_plates(const pmo_plate_instance* inst, int32_t n_inst, const uint32
     P.wth = run->wth; P.roulette_m = run->roulette_m; P.inv_m = 1.f / run->roulette_m;
     P.flags = run->flags;
     P.max_events = kMaxEvents;
+    if (run->quorum < 0 || run->quorum > 64) return fail(PMO_ERR_INVALID, "quorum %d outside [0, 64]", run->quorum);
+    P.quorum = run->quorum > 0 ? (uint32_t)run->quorum : (uint32_t)kDefaultQuorum;
     P.scalars = tal->scalars;
     P.events = reinterpret_cast<unsigned long long*>(tal->events);
     P.nr = tal->nr; P.dr = tal->dr;
@@ -522,8 +524,10 @@ int pmo_walk_plates(const pmo_plate_instance* inst, int32_t n_inst, const uint32
     P.tt_r = tal->nr > 0 ? tal->tt_r : nullptr;
     P.work = reinterpret_cast<unsigned long long*>(workspace);
 
+    const bool layer = (run->flags & PMO_F_LAYER_STEP) != 0u;
     int nb = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::plate_batch_kernel<float>, bs, 0);
+    hipError_t e = layer ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::plate_batch_kernel<float, true>, bs, 0)
+                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::plate_batch_kernel<float, false>, bs, 0);
     if (e != hipSuccess) return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     if (nb < 1) nb = 1;
     unsigned long long grid = run->grid_blocks > 0 ? (unsigned long long)run->grid_blocks
@@ -535,7 +539,10 @@ int pmo_walk_plates(const pmo_plate_instance* inst, int32_t n_inst, const uint32
     hipLaunchKernelGGL(pmo::zero_u64_kernel, dim3(1), dim3(64), 0, s, P.work, 4);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "zero workspace launch");
-    hipLaunchKernelGGL(pmo::plate_batch_kernel<float>, dim3((unsigned)grid), dim3(bs), 0, s, P);
+    if (layer)
+        hipLaunchKernelGGL((pmo::plate_batch_kernel<float, true>), dim3((unsigned)grid), dim3(bs), 0, s, P);
+    else
+        hipLaunchKernelGGL((pmo::plate_batch_kernel<float, false>), dim3((unsigned)grid), dim3(bs), 0, s, P);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "batched walk launch");
     return PMO_OK;

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), n
         assert n in _abi.SIGNATURES, n
-    assert L.pmo_abi_version() == 1
+    assert L.pmo_abi_version() == _abi.ABI_VERSION == 2
     assert L.pmo_workspace_bytes() >= 8
 
 
@@ -38,7 +38,7 @@ PROBE = r"""
 #define O(t, f) printf(#t "." #f " %zu\n", offsetof(t, f));
 int main(void) {
   S(pmo_volume) S(pmo_source) S(pmo_run) S(pmo_tallies) S(pmo_plate_instance) S(pmo_batch_tallies) S(pmo_diff)
-  O(pmo_diff, mismatch) O(pmo_plate_instance, n_packets) O(pmo_run, key_dev) O(pmo_run, quorum)
+  O(pmo_diff, mismatch) O(pmo_plate_instance, n_packets) O(pmo_plate_instance, layers) O(pmo_run, key_dev) O(pmo_run, quorum)
   O(pmo_volume, optics) O(pmo_source, w0) O(pmo_run, packets_per_lane) O(pmo_tallies, exit_state)
   O(pmo_tallies, ring_w) O(pmo_tallies, line_z) O(pmo_run, flags) O(pmo_run, window)
   printf("PMO_S_COUNT %d\nPMO_EV_COUNT %d\n", PMO_S_COUNT, PMO_EV_COUNT);
@@ -68,6 +68,7 @@ def test_ctypes_layout_matches_header(tmp_path):
     assert int(out["pmo_run.quorum"]) == _abi.PmoRun.quorum.offset
     assert int(out["pmo_diff.mismatch"]) == _abi.PmoDiff.mismatch.offset
     assert int(out["pmo_plate_instance.n_packets"]) == _abi.PmoPlateInstance.n_packets.offset
+    assert int(out["pmo_plate_instance.layers"]) == _abi.PmoPlateInstance.layers.offset
     assert int(out["pmo_tallies.exit_state"]) == _abi.PmoTallies.exit_state.offset
     assert int(out["pmo_tallies.ring_w"]) == _abi.PmoTallies.ring_w.offset
     assert int(out["pmo_tallies.line_z"]) == _abi.PmoTallies.line_z.offset

@@ -61,5 +61,21 @@ def test_slab_sweep_distribution():
 
 
 def test_instance_struct_layout():
-    assert C.sizeof(_abi.PmoPlateInstance) == 80
+    assert C.sizeof(_abi.PmoPlateInstance) == 96
     assert _abi.PmoPlateInstance.n_packets.offset == 72
+    assert _abi.PmoPlateInstance.layers.offset == 80
+
+
+def test_layer_table_matches_z_labels():
+    from pymonteopt_amd.batched import Plate
+    p = Plate([0.1, 0.4, 0.25], [0.5, 0.05, 0.1], [30.0, 20.0, 10.0], [0.8, 0.9, 0.9], [1.44, 1.40, 1.37],
+              voxel_space=0.05, x_size=1.05, y_size=1.05)
+    col, tab = p.z_labels(), p.layer_table()
+    l = p.voxel_space
+    assert tab.shape == (3, 4)
+    assert tab[0, 0] == 0.0 and abs(tab[-1, 1] - l * (col.size - 2)) < 1e-6
+    for j in range(tab.shape[0]):
+        z0, z1, lab = int(round(tab[j, 0] / l)) + 1, int(round(tab[j, 1] / l)) + 1, int(tab[j, 2])
+        assert np.all(col[z0:z1] == lab)
+        if j + 1 < tab.shape[0]:
+            assert tab[j, 1] == tab[j + 1, 0] and int(tab[j + 1, 2]) != lab

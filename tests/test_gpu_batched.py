@@ -19,13 +19,18 @@ def plates():
     ]
 
 
-def test_batched_matches_plain_walks():
+@pytest.mark.parametrize("layer_step", [False, True])
+def test_batched_matches_plain_walks(layer_step):
+    """Voxel stepping reproduces each plain walk to rounding of the FP64 sums. Layer
+    stepping uses the same draws and differs only by FP32 rounding of positions and of
+    the optical depth left, which flips a packet's history only rarely: per-instance
+    totals agree to a few 1e-4 of a packet's weight per packet."""
     import torch
     from pymonteopt_amd.batched import PlateSweep
     from pymonteopt_amd.geometry import Source
     from pymonteopt_amd.runtime import TallySpec, Walker
     ps = plates()
-    sw = PlateSweep(ps, nr=20, dr=0.1, batch=128)
+    sw = PlateSweep(ps, nr=20, dr=0.1, batch=128, layer_step=layer_step)
     sw.run(seed=5, stream_id=40)
     torch.cuda.synchronize()
     res = sw.results()
@@ -38,9 +43,16 @@ def test_batched_matches_plain_walks():
         torch.cuda.synchronize()
         r = tal.to_host()
         s = r["scalars"]
+        N = p.n_packets
         for j, k in enumerate(["Rd", "Tt", "other", "A", "kill", "gain", "w0"]):
-            assert res["scalars"][i, j] == pytest.approx(s[k], rel=1e-9, abs=1e-9), (i, k)
-        assert np.allclose(res["rd_r_raw"][i], r["rd_r"], rtol=1e-9, atol=1e-12)
+            if layer_step:
+                assert abs(res["scalars"][i, j] - s[k]) / N < 3e-4, (i, k, res["scalars"][i, j], s[k])
+            else:
+                assert res["scalars"][i, j] == pytest.approx(s[k], rel=1e-9, abs=1e-9), (i, k)
+        if layer_step:
+            assert np.abs(res["rd_r_raw"][i] - r["rd_r"]).sum() / N < 1e-3
+        else:
+            assert np.allclose(res["rd_r_raw"][i], r["rd_r"], rtol=1e-9, atol=1e-12)
         bal = sum(res["scalars"][i, j] for j in (0, 1, 2, 3, 4)) - res["scalars"][i, 5]
         assert abs(bal - res["scalars"][i, 6]) < 1e-9 * res["scalars"][i, 6]
 
