@@ -145,6 +145,52 @@ def adjoint_leg(n_packets=10_000_000):
     return out
 
 
+def obd_leg(n_packets=10_000_000, sample=20000):
+    """OBD receiver scan (260 focus steps, reference defaults) of the exit states of a 1e7
+    packet walk of the skin plate, traced on the device straight from the walk's buffer.
+    Also returns (not printed) a host copy of the first exits, for the CPU baseline."""
+    import torch
+    from pymonteopt_amd import presets
+    from pymonteopt_amd.optics import OBD
+    from pymonteopt_amd.runtime import TallySpec, Walker
+    wl = presets.skin_plate()
+    wk = Walker(wl.labels, wl.pitch, wl.optics)
+    tal = wk.tallies(TallySpec(absorbed=False, exit_state=True), n_packets)
+    wk.walk(wl.source, n_packets, 7, tal)
+    o = OBD()
+    o.trace_exit_states(tal.exit_state, seed=1)          # warm-up
+    torch.cuda.synchronize()
+    e0, e1 = _events()
+    e0.record()
+    I = o.trace_exit_states(tal.exit_state, seed=2)
+    e1.record()
+    e1.synchronize()
+    t = e0.elapsed_time(e1) * 1e-3
+    es = tal.exit_state.reshape(7, n_packets)
+    n_rd = int((es[5] < 0).sum())
+    steps = o.steps().size
+    res = {"packets_walked": n_packets, "exits_traced": n_rd, "focus_steps": steps, "seconds": t,
+           "traces_per_s": n_rd * steps / t, "peak_Z": float(o.steps()[int(torch.argmax(I))])}
+    return res, (o.params, es[:, :4 * sample].cpu().numpy(), sample)
+
+
+def cpu_baseline_obd(params, host_es, sample):
+    """The oracle's numpy receiver (float64, one process) on `sample` exits toward the
+    receiver from the same walk, over the same 260 focus steps."""
+    import numpy as np
+    from oracle import obd as ob
+    h = host_es.astype(np.float64)
+    q, d, wt, idx = ob.select(params, h[0:3], h[3:6], h[6])
+    q, d, wt, idx = q[:, :sample], d[:, :sample], wt[:sample], idx[:sample]
+    Zs = ob.steps(params)
+    t0 = time.perf_counter()
+    for i, Z in enumerate(Zs):
+        ob.trace_step(params, Z, q, d, wt, u=ob.step_uniforms(i, idx, 2, 0))
+    tc = time.perf_counter() - t0
+    return {"value": wt.size * Zs.size / tc, "unit": "traces/s", "cores": 1, "kind": "port",
+            "sample": f"{wt.size} exits x {Zs.size} focus steps, numpy float64, {tc:.1f} s"}
+
+
 def measure_peaks():
     """Re-measures the two peaks with libpmo's probes: FP64 global atomic adds per second
     (pseudo-random addresses in 32 MiB) and FP32 FMA lane-ops per second."""
@@ -336,8 +382,12 @@ def main():
     if not args.no_extra:
         out["batched"] = batched_leg()
         out["adjoint"] = adjoint_leg()
+        out["obd"], obd_sample = obd_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl)
+        if not args.no_extra:
+            out["obd"]["cpu_baseline"] = cpu_baseline_obd(*obd_sample)
+            out["obd"]["gpu_vs_cpu"] = out["obd"]["traces_per_s"] / out["obd"]["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
 
 

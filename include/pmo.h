@@ -260,6 +260,37 @@ int pmo_probe_valu(float* out, int32_t blocks, int32_t iters, void* stream);
 int64_t pmo_plate_schedule(const uint64_t* n_packets, int32_t n_inst, int32_t batch, uint32_t* sched,
                            int64_t cap);
 
+/* ---- OBD receiver ------------------------------------------------------------------- */
+/* The receiver of pymopt/optics/_classes.py:381-567 (class OBD; SideOBD through x_shift):
+ * lens 1 (plano face toward the sample), slit 1, the LD mount, slit 2, lens 2 (convex face
+ * toward the sample) and the photodiode, scanned over focus offsets Z. Lengths in mm. */
+#define PMO_OBD_STEP_FIELDS 12  /* floats per row of `steps`: z_lens1, c_lens1 (lens-1 sphere
+                                   centre), z_slit1, z_lfp (LD mount front), z_slit2, z_lens2,
+                                   c_lens2, z_pd, x_shift (SideOBD lateral shift), 3 unused;
+                                   computed as OBD.opticalUnit does (:454-460)              */
+typedef struct pmo_obd {
+    float r_1, n_1, outer_r1;      /* lens 1: surface radius r_1, index, aperture outerD_1/2    */
+    float r_2, n_2, outer_r2;      /* lens 2                                                    */
+    float slit_r_in, slit_r_out;   /* annulus slit_D/2 -/+ slit_width/2 (Slit.d_in, d_out)      */
+    float slit_thickness;
+    float pd_r;                    /* photodiode radius d_pd/2                                  */
+    int32_t ld_fix_part;           /* apply the LD mount cut                                    */
+    float lfp_angle;               /* LdFixingPart.angle as the reference writes it: 68/180     */
+    float lfp_thickness, lfp_width;/* 5 and 6 mm in the reference                               */
+    int32_t inversion;             /* trace v_z > 0 exits, mirrored in y (params 'inversion')   */
+    float side;                    /* side view of a block this wide, 0 = off (params 'side')   */
+    int32_t expect;                /* 1: weight by the Fresnel transmission instead of drawing  */
+    uint32_t seed, stream;         /* Philox key of the Fresnel draws                           */
+} pmo_obd;
+
+/* Adds to intensity[s] the weight of the exit packets that reach the photodiode at focus
+ * step s (OBD.opticalAnalysisMeth, :491-525). p and v are device [3][stride] rows x, y, z
+ * (the layout of pmo_tallies.exit_state rows 0-5), w is device [n]; the packets taken are
+ * those with v_z < 0 (v_z > 0 with inversion). steps is device [n_steps][PMO_OBD_STEP_FIELDS],
+ * 1 <= n_steps <= 1024, and intensity device [n_steps] (FP64, zeroed by the caller). */
+int pmo_obd_trace(const pmo_obd* cfg, const float* p, const float* v, const float* w, int64_t n, int64_t stride,
+                  const float* steps, int32_t n_steps, double* intensity, void* stream);
+
 /* Many plates in one persistent launch. Packet j of instance i draws from Philox key
  * (seed, stream + i) with packet id j, i.e. exactly the draws of a plain pmo_walk of that
  * plate with stream + i, so the sweep reproduces the sequential runs it replaces: one

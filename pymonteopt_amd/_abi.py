@@ -78,6 +78,18 @@ class PmoBatchTallies(C.Structure):
                 ("rd_r", C.c_void_p), ("tt_r", C.c_void_p)]
 
 
+PMO_OBD_STEP_FIELDS = 12
+
+
+class PmoObd(C.Structure):
+    _fields_ = [("r_1", C.c_float), ("n_1", C.c_float), ("outer_r1", C.c_float), ("r_2", C.c_float),
+                ("n_2", C.c_float), ("outer_r2", C.c_float), ("slit_r_in", C.c_float), ("slit_r_out", C.c_float),
+                ("slit_thickness", C.c_float), ("pd_r", C.c_float), ("ld_fix_part", C.c_int32),
+                ("lfp_angle", C.c_float), ("lfp_thickness", C.c_float), ("lfp_width", C.c_float),
+                ("inversion", C.c_int32), ("side", C.c_float), ("expect", C.c_int32), ("seed", C.c_uint32),
+                ("stream", C.c_uint32)]
+
+
 class PmoDiff(C.Structure):
     _fields_ = [("mua_field", C.c_void_p), ("nr", C.c_int32), ("dr", C.c_float), ("rd_r", C.c_void_p),
                 ("jac", C.c_void_p), ("exit_rec", C.c_void_p), ("coef", C.c_void_p), ("grad_mua", C.c_void_p),
@@ -97,6 +109,8 @@ SIGNATURES = {
                                C.c_void_p, C.POINTER(PmoDiff), C.c_void_p, C.c_void_p], C.c_int),
     "pmo_walk_adjoint": ([C.POINTER(PmoVolume), C.POINTER(PmoSource), C.POINTER(PmoRun), C.POINTER(PmoDiff),
                           C.c_void_p, C.c_void_p], C.c_int),
+    "pmo_obd_trace": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int32,
+                       C.c_void_p, C.c_void_p], C.c_int),
     "pmo_probe_atomics": ([C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_void_p], C.c_int),
     "pmo_probe_valu": ([C.c_void_p, C.c_int32, C.c_int32, C.c_void_p], C.c_int),
     "pmo_plate_schedule": ([C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int64], C.c_int64),

@@ -4,6 +4,7 @@
 // nothing: the only per-call scratch is the caller's workspace.
 #include <hip/hip_runtime.h>
 
+#include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -13,6 +14,7 @@
 #include "pmo.h"
 #include "adjoint.hpp"
 #include "batch.hpp"
+#include "obd.hpp"
 #include "walk.hpp"
 
 namespace {
@@ -546,6 +548,62 @@ int pmo_walk_plates(const pmo_plate_instance* inst, int32_t n_inst, const uint32
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "batched walk launch");
     return PMO_OK;
+}
+
+int pmo_obd_trace(const pmo_obd* cfg, const float* p, const float* v, const float* w, int64_t n, int64_t stride,
+                  const float* steps, int32_t n_steps, double* intensity, void* stream)
+{
+    g_err[0] = '\0';
+    static_assert(sizeof(pmo::ObdStep) == PMO_OBD_STEP_FIELDS * sizeof(float), "OBD step layout");
+    if (!cfg || !steps || !intensity) return fail(PMO_ERR_INVALID, "null argument");
+    if (n < 0 || stride < n) return fail(PMO_ERR_INVALID, "need 0 <= n <= stride");
+    if (n > 0 && (!p || !v || !w)) return fail(PMO_ERR_INVALID, "null exit-state array");
+    if (n_steps < 1 || n_steps > 1024) return fail(PMO_ERR_INVALID, "n_steps %d outside [1, 1024]", n_steps);
+    if (!(cfg->n_1 > 1.f) || !(cfg->n_2 > 1.f)) return fail(PMO_ERR_INVALID, "lens indices must exceed 1");
+    if (!(cfg->r_1 > 0.f) || !(cfg->r_2 > 0.f) || !(cfg->outer_r1 > 0.f) || !(cfg->outer_r2 > 0.f) ||
+        cfg->outer_r1 >= cfg->r_1 || cfg->outer_r2 >= cfg->r_2)
+        return fail(PMO_ERR_INVALID, "lens radii: need 0 < aperture < surface radius");
+    if (!(cfg->slit_r_out > cfg->slit_r_in) || !(cfg->pd_r > 0.f) || cfg->slit_thickness < 0.f)
+        return fail(PMO_ERR_INVALID, "bad slit or photodiode size");
+    if (n == 0) return PMO_OK;
+    DevInfo d;
+    int rc = device_info(&d);
+    if (rc != PMO_OK) return rc;
+
+    pmo::ObdParams P;
+    memset(&P, 0, sizeof(P));
+    P.p = p; P.v = v; P.w = w;
+    P.n = (unsigned long long)n; P.stride = (unsigned long long)stride;
+    P.steps = reinterpret_cast<const pmo::ObdStep*>(steps);
+    P.n_steps = n_steps;
+    P.r1 = cfg->r_1; P.n1 = cfg->n_1; P.outer1_sq = cfg->outer_r1 * cfg->outer_r1;
+    P.r2 = cfg->r_2; P.n2 = cfg->n_2; P.outer2_sq = cfg->outer_r2 * cfg->outer_r2;
+    P.d_in = cfg->slit_r_in; P.d_out = cfg->slit_r_out; P.slit_th = cfg->slit_thickness;
+    P.pd_r_sq = cfg->pd_r * cfg->pd_r;
+    P.ld_fix = cfg->ld_fix_part;
+    // LD mount half-planes (LdFixingPart, _classes.py:337-350)
+    const double ang = cfg->lfp_angle, wd = cfg->lfp_width;
+    P.lfp_a1 = (float)(1.0 / tan(ang) - 1.0 / (sin(ang) * cos(ang)));
+    P.lfp_b1 = (float)(wd / cos(ang));
+    P.lfp_a2 = (float)((cos(ang) + 1.0) / sin(ang));
+    P.lfp_b2 = (float)(-wd);
+    P.lfp_b3 = (float)(-wd);
+    P.lfp_th = cfg->lfp_thickness;
+    P.inversion = cfg->inversion;
+    P.side = cfg->side;
+    P.expect = cfg->expect;
+    P.seed = cfg->seed; P.stream = cfg->stream;
+    P.out = intensity;
+
+    const int bs = 256;
+    const size_t lds = (size_t)n_steps * (sizeof(pmo::ObdStep) + sizeof(double));
+    unsigned long long blocks = ((unsigned long long)n + bs - 1) / bs;
+    const unsigned long long cap = (unsigned long long)d.n_cu * 8ull;
+    if (blocks > cap) blocks = cap;
+    hipLaunchKernelGGL(pmo::obd_kernel, dim3((unsigned)blocks), dim3(bs), lds, reinterpret_cast<hipStream_t>(stream),
+                       P);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PMO_OK : hip_fail(e, "OBD launch");
 }
 
 }  // extern "C"

@@ -37,11 +37,12 @@ PROBE = r"""
 #define S(t) printf(#t " %zu\n", sizeof(t));
 #define O(t, f) printf(#t "." #f " %zu\n", offsetof(t, f));
 int main(void) {
-  S(pmo_volume) S(pmo_source) S(pmo_run) S(pmo_tallies) S(pmo_plate_instance) S(pmo_batch_tallies) S(pmo_diff)
+  S(pmo_volume) S(pmo_source) S(pmo_run) S(pmo_tallies) S(pmo_plate_instance) S(pmo_batch_tallies) S(pmo_diff) S(pmo_obd)
   O(pmo_diff, mismatch) O(pmo_plate_instance, n_packets) O(pmo_plate_instance, layers) O(pmo_run, key_dev) O(pmo_run, quorum)
   O(pmo_volume, optics) O(pmo_source, w0) O(pmo_run, packets_per_lane) O(pmo_tallies, exit_state)
   O(pmo_tallies, ring_w) O(pmo_tallies, line_z) O(pmo_run, flags) O(pmo_run, window)
-  printf("PMO_S_COUNT %d\nPMO_EV_COUNT %d\n", PMO_S_COUNT, PMO_EV_COUNT);
+  O(pmo_obd, stream) O(pmo_obd, side)
+  printf("PMO_S_COUNT %d\nPMO_EV_COUNT %d\nPMO_OBD_STEP_FIELDS %d\n", PMO_S_COUNT, PMO_EV_COUNT, PMO_OBD_STEP_FIELDS);
   return 0;
 }
 """
@@ -57,7 +58,7 @@ def test_ctypes_layout_matches_header(tmp_path):
     for name, cls in (("pmo_volume", _abi.PmoVolume), ("pmo_source", _abi.PmoSource),
                       ("pmo_run", _abi.PmoRun), ("pmo_tallies", _abi.PmoTallies),
                       ("pmo_plate_instance", _abi.PmoPlateInstance), ("pmo_batch_tallies", _abi.PmoBatchTallies),
-                      ("pmo_diff", _abi.PmoDiff)):
+                      ("pmo_diff", _abi.PmoDiff), ("pmo_obd", _abi.PmoObd)):
         assert int(out[name]) == C.sizeof(cls), name
     assert int(out["pmo_volume.optics"]) == _abi.PmoVolume.optics.offset
     assert int(out["pmo_source.w0"]) == _abi.PmoSource.w0.offset
@@ -73,6 +74,9 @@ def test_ctypes_layout_matches_header(tmp_path):
     assert int(out["pmo_tallies.ring_w"]) == _abi.PmoTallies.ring_w.offset
     assert int(out["pmo_tallies.line_z"]) == _abi.PmoTallies.line_z.offset
     assert int(out["PMO_S_COUNT"]) == _abi.PMO_S_COUNT
+    assert int(out["pmo_obd.stream"]) == _abi.PmoObd.stream.offset
+    assert int(out["pmo_obd.side"]) == _abi.PmoObd.side.offset
+    assert int(out["PMO_OBD_STEP_FIELDS"]) == _abi.PMO_OBD_STEP_FIELDS
     assert int(out["PMO_EV_COUNT"]) == _abi.PMO_EV_COUNT
 
 
