@@ -123,3 +123,20 @@ def test_start_and_result_frame(tmp_path):
     assert len(res) == 40 and res["int"].sum() > 0
     o.save_result(str(tmp_path / "scan"))
     assert (tmp_path / "scan_opt.csv").exists()
+
+
+def test_model_result_feeds_obd():
+    from pymonteopt_amd.optics import OBD
+    from pymonteopt_amd.voxel import VoxelPlateModel
+    m = VoxelPlateModel(nPh=200000, exit_state=True)
+    m.build(thickness=[0.1, 2.0, 4.9], ma=[0.5, 0.05, 0.02], ms=[30, 20, 12], g=[0.8, 0.9, 0.9],
+            n=[1.44, 1.40, 1.44], n_air=1.0, x_size=10.05, y_size=10.05, voxel_space=0.05)
+    m.start(rand_seed=4)
+    o = OBD()
+    o.set_monte_data(m.get_result())
+    assert o.nPh == 200000 and o.data["p"].shape == (3, 200000)
+    o.set_params(**SCAN)
+    o.start(seed=1, expect=True)
+    res = o.get_result()
+    ref = oracle_curve(o, o.data["p"], o.data["v"], o.data["w"], "expect")
+    assert np.all(np.abs(res["int"].to_numpy() - ref) <= 4 * float(np.max(o.data["w"])) + 1e-4 * ref)

@@ -35,6 +35,9 @@ def test_params_contract_and_indices():
     assert o.nPh == 123
     with pytest.raises(KeyError):
         o.set_monte_data(photons=1)
+    # a model's whole result dict: the exit states and nPh are taken, the tallies ignored
+    o.set_monte_data({"p": 1, "v": 2, "w": 3, "nPh": 7, "Rd": 0.4, "Rd_r": [0.1]})
+    assert (o.data["p"], o.data["v"], o.data["w"], o.nPh) == (1, 2, 3, 7)
 
 
 def test_step_table_matches_the_oracle_geometry():
