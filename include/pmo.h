@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define PMO_ABI_VERSION 2
+#define PMO_ABI_VERSION 3
 
 /* ---- error codes ---------------------------------------------------------------- */
 #define PMO_OK 0
@@ -150,6 +150,13 @@ typedef struct pmo_tallies {
     double* ring_w;         /* device [n_rings] or NULL                                        */
     float* exit_state;      /* device [7][n_packets] (x, y, z, vx, vy, vz, w) in the reference's
                                get_result encoding, indexed by packet id - first_packet, or NULL */
+    int32_t rz_nr, rz_nz;   /* internal fluence bins r in [i*rz_dr, (i+1)*rz_dr), z in
+                               [k*rz_dz, (k+1)*rz_dz), r = hypot(x, y) and z of the interaction point */
+    float rz_dr, rz_dz;
+    double* fluence_rz;     /* device [rz_nr][rz_nz]: sum of deposit / mu_a at each interaction,
+                               the r-z fluence pymopt/fluence/intarnal_fluence.py Fluence2D intends
+                               (pymopt/voxel_unverified/_classes.py:394-399), or NULL. Held in LDS
+                               per block: rz_nr*rz_nz*8 B counts against the 64 KiB budget     */
 } pmo_tallies;
 
 /* ---- batched plates ----------------------------------------------------------------- */
@@ -290,6 +297,16 @@ typedef struct pmo_obd {
  * 1 <= n_steps <= 1024, and intensity device [n_steps] (FP64, zeroed by the caller). */
 int pmo_obd_trace(const pmo_obd* cfg, const float* p, const float* v, const float* w, int64_t n, int64_t stride,
                   const float* steps, int32_t n_steps, double* intensity, void* stream);
+
+/* ---- fluence binning ---------------------------------------------------------------- */
+/* Fluence2D / Fluence3D.saveFluesnce of pymopt/fluence/intarnal_fluence.py (:12-152) with
+ * each weight kept with its own point (the reference sorts the points but not the weights,
+ * :35-37 and :113-115). Adds w[j] to the bin of point (p[0][j], p[1][j], p[2][j]); p is
+ * device [3][stride], w device [n]. dims = 2: out [nr][nz], r = hypot(x, y) in
+ * [i*dr, (i+1)*dr), z in [k*dz, (k+1)*dz). dims = 3: out [2nr][2nr][nz], x and y in
+ * [(i-nr)*dr, (i-nr+1)*dr). Points outside every bin are skipped. out is FP64 (accumulated). */
+int pmo_fluence_bin(const float* p, int64_t stride, const float* w, int64_t n, int32_t dims, int32_t nr, float dr,
+                    int32_t nz, float dz, double* out, void* stream);
 
 /* Many plates in one persistent launch. Packet j of instance i draws from Philox key
  * (seed, stream + i) with packet id j, i.e. exactly the draws of a plain pmo_walk of that

@@ -77,6 +77,12 @@ typedef struct og_tallies {
        sum of w*L_j and w*n_j (L_j path length, n_j interactions in label j) */
     double* rd_wl;         /* [nr][n_labels] or NULL                                 */
     double* rd_wn;         /* [nr][n_labels] or NULL                                 */
+    /* internal fluence in r-z bins [i*rz_dr, (i+1)*rz_dr) x [k*rz_dz, (k+1)*rz_dz): sum of
+       deposit / mu_a at each interaction's position (the intent of
+       pymopt/fluence/intarnal_fluence.py Fluence2D, pymopt/voxel_unverified/_classes.py:394-399) */
+    int32_t rz_nr, rz_nz;
+    double rz_dr, rz_dz;
+    double* fluence_rz;    /* [rz_nr][rz_nz] or NULL                                 */
 } og_tallies;
 
 /* event trace of one packet (type, voxel, position, direction, weight) */

@@ -49,7 +49,9 @@ class OgTallies(C.Structure):
                 ("nl", C.c_int32), ("dl", C.c_double), ("line_y", C.c_double),
                 ("line_z", C.c_void_p), ("n_rings", C.c_int32), ("rings", C.c_void_p),
                 ("ring_w", C.c_void_p), ("exit_state", C.c_void_p),
-                ("rd_wl", C.c_void_p), ("rd_wn", C.c_void_p)]
+                ("rd_wl", C.c_void_p), ("rd_wn", C.c_void_p),
+                ("rz_nr", C.c_int32), ("rz_nz", C.c_int32), ("rz_dr", C.c_double), ("rz_dz", C.c_double),
+                ("fluence_rz", C.c_void_p)]
 
 
 class OgEvent(C.Structure):
@@ -169,7 +171,7 @@ def _source(src):
 
 def walk(labels, pitch, mua, mus, g, n, source, n_packets, seed, *, stream=0, first_packet=0,
          wth=1e-4, roulette_m=10.0, flags=0, nr=0, dr=0.1, na=0, nl=0, dl=0.1, line_y=5.0,
-         rings=None, absorbed=True, exit_state=False, path_stats=False, precision="f64",
+         rings=None, absorbed=True, exit_state=False, path_stats=False, rz=None, precision="f64",
          threads=0):
     """Run the oracle walk; returns the raw tally dictionary (unnormalised sums)."""
     vol, keep = _volume(labels, pitch, mua, mus, g, n)
@@ -199,6 +201,10 @@ def walk(labels, pitch, mua, mus, g, n, source, n_packets, seed, *, stream=0, fi
         bufs["ring_w"] = np.zeros(ring_arr.size // 2, np.float64)
     if exit_state:
         bufs["exit_state"] = np.zeros(7 * int(n_packets), np.float32)
+    if rz:
+        rz_nr, rz_dr, rz_nz, rz_dz = rz
+        bufs["fluence_rz"] = np.zeros(int(rz_nr) * int(rz_nz), np.float64)
+        t.rz_nr, t.rz_nz, t.rz_dr, t.rz_dz = int(rz_nr), int(rz_nz), float(rz_dr), float(rz_dz)
     t.absorbed = _ptr(bufs.get("absorbed"))
     t.nr, t.dr = int(nr), float(dr)
     t.rd_r, t.tt_r, t.rd_top_r = _ptr(bufs.get("rd_r")), _ptr(bufs.get("tt_r")), _ptr(bufs.get("rd_top_r"))
@@ -211,6 +217,7 @@ def walk(labels, pitch, mua, mus, g, n, source, n_packets, seed, *, stream=0, fi
     t.ring_w = _ptr(bufs.get("ring_w"))
     t.exit_state = _ptr(bufs.get("exit_state"))
     t.rd_wl, t.rd_wn = _ptr(bufs.get("rd_wl")), _ptr(bufs.get("rd_wn"))
+    t.fluence_rz = _ptr(bufs.get("fluence_rz"))
     fn = lib().og_walk_f64 if precision == "f64" else lib().og_walk_f32
     rc = fn(C.byref(vol), C.byref(src), C.byref(run), C.byref(t))
     if rc != 0:
@@ -223,6 +230,8 @@ def walk(labels, pitch, mua, mus, g, n, source, n_packets, seed, *, stream=0, fi
         out["absorbed"] = out["absorbed"].reshape(vol.nx, vol.ny, vol.nz)
     if "exit_state" in out:
         out["exit_state"] = out["exit_state"].reshape(7, int(n_packets))
+    if rz:
+        out["fluence_rz"] = out["fluence_rz"].reshape(int(rz[0]), int(rz[2]))
     if path_stats and nr > 0:
         out["rd_wl"] = out["rd_wl"].reshape(nr, vol.n_labels)
         out["rd_wn"] = out["rd_wn"].reshape(nr, vol.n_labels)

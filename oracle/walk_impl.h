@@ -23,6 +23,7 @@ typedef struct FN(ctx) {
     double* ring_w;
     double* rd_wl;
     double* rd_wn;
+    double* frz;
 } FN(ctx);
 
 static inline REAL FN(rsgn)(REAL x) { return x > 0 ? (REAL)1 : (x < 0 ? (REAL)-1 : (REAL)0); }
@@ -310,6 +311,17 @@ static int FN(packet)(const og_volume* V, const og_source* S, const og_run* R, c
             C->ev[1]++;
             C->scal[3] += dep;
             if (C->absorbed) C->absorbed[FN(lin)(V, a[0], a[1], a[2])] += dep;
+            if (C->frz && V->mua[lab] > 0) {                 /* r-z fluence at the interaction */
+                double l64 = V->pitch;
+                int cx = (V->nx - 1) / 2, cy = (V->ny - 1) / 2;
+                double x = l64 * (a[0] - cx) + (double)p[0];
+                double y = l64 * (a[1] - cy) + (double)p[1];
+                double z = l64 * (a[2] - 1) + (double)p[2] + l64 / 2;
+                int64_t ir = (int64_t)floor(sqrt(x * x + y * y) / T->rz_dr);
+                int64_t iz = (int64_t)floor(z / T->rz_dz);
+                if (ir >= 0 && ir < T->rz_nr && iz >= 0 && iz < T->rz_nz)
+                    C->frz[ir * T->rz_nz + iz] += dep / V->mua[lab];
+            }
         }
         if (w <= (REAL)R->wth) {                            /* roulette, :132-141 */
             if (C) C->ev[4]++;
@@ -375,6 +387,8 @@ int FN(og_walk)(const og_volume* V, const og_source* S, const og_run* R, og_tall
         C->ring_w = (T->n_rings > 0 && T->ring_w) ? (double*)calloc((size_t)T->n_rings, sizeof(double)) : NULL;
         C->rd_wl = (T->nr > 0 && T->rd_wl) ? (double*)calloc((size_t)T->nr * nl, sizeof(double)) : NULL;
         C->rd_wn = (T->nr > 0 && T->rd_wn) ? (double*)calloc((size_t)T->nr * nl, sizeof(double)) : NULL;
+        C->frz = (T->rz_nr > 0 && T->rz_nz > 0 && T->fluence_rz)
+                     ? (double*)calloc((size_t)T->rz_nr * T->rz_nz, sizeof(double)) : NULL;
     }
     const int64_t n = (int64_t)R->n_packets;
 #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64)
@@ -398,6 +412,10 @@ int FN(og_walk)(const og_volume* V, const og_source* S, const og_run* R, og_tall
         MERGE(ring_w, T->n_rings)
         MERGE(rd_wl, (int64_t)T->nr * nl)
         MERGE(rd_wn, (int64_t)T->nr * nl)
+        if (C->frz) {
+            for (int64_t q = 0; q < (int64_t)T->rz_nr * T->rz_nz; q++) T->fluence_rz[q] += C->frz[q];
+            free(C->frz);
+        }
 #undef MERGE
     }
     free(ctxs);

@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpmo.so")
 
-ABI_VERSION = 2          # PMO_ABI_VERSION of include/pmo.h
+ABI_VERSION = 3          # PMO_ABI_VERSION of include/pmo.h
 
 PMO_OK = 0
 PMO_ERR_INVALID = -1
@@ -62,7 +62,8 @@ class PmoTallies(C.Structure):
                 ("na", C.c_int32), ("rd_a", C.c_void_p), ("tt_a", C.c_void_p),
                 ("nl", C.c_int32), ("dl", C.c_float), ("line_y", C.c_float), ("line_z", C.c_void_p),
                 ("n_rings", C.c_int32), ("rings", C.c_void_p), ("ring_w", C.c_void_p),
-                ("exit_state", C.c_void_p)]
+                ("exit_state", C.c_void_p), ("rz_nr", C.c_int32), ("rz_nz", C.c_int32), ("rz_dr", C.c_float),
+                ("rz_dz", C.c_float), ("fluence_rz", C.c_void_p)]
 
 
 class PmoPlateInstance(C.Structure):
@@ -109,6 +110,8 @@ SIGNATURES = {
                                C.c_void_p, C.POINTER(PmoDiff), C.c_void_p, C.c_void_p], C.c_int),
     "pmo_walk_adjoint": ([C.POINTER(PmoVolume), C.POINTER(PmoSource), C.POINTER(PmoRun), C.POINTER(PmoDiff),
                           C.c_void_p, C.c_void_p], C.c_int),
+    "pmo_fluence_bin": ([C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_int32,
+                         C.c_float, C.c_void_p, C.c_void_p], C.c_int),
     "pmo_obd_trace": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int32,
                        C.c_void_p, C.c_void_p], C.c_int),
     "pmo_probe_atomics": ([C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_void_p], C.c_int),

@@ -70,10 +70,11 @@ int device_info(DevInfo* out)
 }
 
 template <typename R>
-int blocks_per_cu(int block, size_t lds, int* out)
+int blocks_per_cu(int block, size_t lds, int* out, bool rz = false)
 {
     int nb = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::walk_kernel<R>, block, lds);
+    hipError_t e = rz ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::walk_kernel<R, true>, block, lds)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::walk_kernel<R, false>, block, lds);
     if (e != hipSuccess) return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     *out = nb > 0 ? nb : 1;
     return PMO_OK;
@@ -185,6 +186,14 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     P.n_rings = tal->n_rings; P.rings = tal->rings;
     P.ring_w = tal->n_rings > 0 ? tal->ring_w : nullptr;
     P.exit_state = tal->exit_state;
+    if (tal->fluence_rz) {
+        if (tal->rz_nr < 1 || tal->rz_nz < 1 || !(tal->rz_dr > 0.f) || !(tal->rz_dz > 0.f))
+            return fail(PMO_ERR_INVALID, "r-z fluence needs rz_nr, rz_nz >= 1 and rz_dr, rz_dz > 0");
+        if ((long long)tal->rz_nr * tal->rz_nz > (1 << 13))
+            return fail(PMO_ERR_UNSUPPORTED, "r-z fluence of %d x %d bins exceeds 8192", tal->rz_nr, tal->rz_nz);
+        P.rz_nr = tal->rz_nr; P.rz_nz = tal->rz_nz; P.rz_dr = tal->rz_dr; P.rz_dz = tal->rz_dz;
+        P.fluence_rz = tal->fluence_rz;
+    }
     P.work = reinterpret_cast<unsigned long long*>(workspace);
     P.key_dev = run->key_dev;
 
@@ -227,6 +236,7 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     P.h_tta = off; off += P.tt_a ? (uint32_t)P.na : 0u;
     P.h_line = off; off += P.line_z ? (uint32_t)P.nl : 0u;
     P.h_ring = off; off += P.ring_w ? (uint32_t)P.n_rings : 0u;
+    P.h_rz = off; off += P.fluence_rz ? (uint32_t)(P.rz_nr * P.rz_nz) : 0u;
     P.h_total = off;
     P.lds_win = (uint32_t)align_up(P.lds_hist + (size_t)P.h_total * sizeof(double), 16);
     const size_t lds = P.lds_win + (size_t)P.win_cells * sizeof(double);
@@ -234,7 +244,8 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
         return fail(PMO_ERR_UNSUPPORTED, "histograms and tally window need %zu B of LDS (limit %zu)", lds, kMaxLds);
 
     int nb = 0;
-    rc = fp64 ? blocks_per_cu<double>(bs, lds, &nb) : blocks_per_cu<float>(bs, lds, &nb);
+    rc = fp64 ? blocks_per_cu<double>(bs, lds, &nb, P.fluence_rz != nullptr)
+              : blocks_per_cu<float>(bs, lds, &nb, P.fluence_rz != nullptr);
     if (rc != PMO_OK) return rc;
     unsigned long long grid = run->grid_blocks > 0 ? (unsigned long long)run->grid_blocks
                                                    : (unsigned long long)nb * (unsigned long long)d.n_cu;
@@ -258,9 +269,15 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "zero workspace launch");
     if (fp64)
-        hipLaunchKernelGGL(pmo::walk_kernel<double>, dim3((unsigned)grid), dim3(bs), lds, s, P);
+        if (P.fluence_rz)
+            hipLaunchKernelGGL((pmo::walk_kernel<double, true>), dim3((unsigned)grid), dim3(bs), lds, s, P);
+        else
+            hipLaunchKernelGGL((pmo::walk_kernel<double, false>), dim3((unsigned)grid), dim3(bs), lds, s, P);
     else
-        hipLaunchKernelGGL(pmo::walk_kernel<float>, dim3((unsigned)grid), dim3(bs), lds, s, P);
+        if (P.fluence_rz)
+            hipLaunchKernelGGL((pmo::walk_kernel<float, true>), dim3((unsigned)grid), dim3(bs), lds, s, P);
+        else
+            hipLaunchKernelGGL((pmo::walk_kernel<float, false>), dim3((unsigned)grid), dim3(bs), lds, s, P);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "walk launch");
     return PMO_OK;
@@ -548,6 +565,28 @@ int pmo_walk_plates(const pmo_plate_instance* inst, int32_t n_inst, const uint32
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "batched walk launch");
     return PMO_OK;
+}
+
+int pmo_fluence_bin(const float* p, int64_t stride, const float* w, int64_t n, int32_t dims, int32_t nr, float dr,
+                    int32_t nz, float dz, double* out, void* stream)
+{
+    g_err[0] = '\0';
+    if (!out) return fail(PMO_ERR_INVALID, "null output");
+    if (n < 0 || stride < n) return fail(PMO_ERR_INVALID, "need 0 <= n <= stride");
+    if (n > 0 && (!p || !w)) return fail(PMO_ERR_INVALID, "null point or weight array");
+    if (dims != 2 && dims != 3) return fail(PMO_ERR_INVALID, "dims must be 2 or 3");
+    if (nr < 1 || nz < 1 || !(dr > 0.f) || !(dz > 0.f)) return fail(PMO_ERR_INVALID, "bad bins");
+    if (n == 0) return PMO_OK;
+    DevInfo d;
+    int rc = device_info(&d);
+    if (rc != PMO_OK) return rc;
+    unsigned long long blocks = ((unsigned long long)n + 255) / 256;
+    if (blocks > (unsigned long long)d.n_cu * 8ull) blocks = (unsigned long long)d.n_cu * 8ull;
+    hipLaunchKernelGGL(pmo::fluence_bin_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), p, (unsigned long long)stride, w, (unsigned long long)n,
+                       dims, nr, dr, nz, dz, out);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PMO_OK : hip_fail(e, "fluence binning launch");
 }
 
 int pmo_obd_trace(const pmo_obd* cfg, const float* p, const float* v, const float* w, int64_t n, int64_t stride,

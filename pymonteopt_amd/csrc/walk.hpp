@@ -36,6 +36,7 @@ template <> struct Mth<float> {
     static __device__ __forceinline__ float fabs(float x) { return __builtin_fabsf(x); }
     static __device__ __forceinline__ float copysign(float a, float b) { return __builtin_copysignf(a, b); }
     static __device__ __forceinline__ float fmax(float a, float b) { return __builtin_fmaxf(a, b); }
+    static __device__ __forceinline__ float floor(float x) { return __builtin_floorf(x); }
 };
 
 template <> struct Mth<double> {
@@ -51,6 +52,7 @@ template <> struct Mth<double> {
     static __device__ __forceinline__ double fabs(double x) { return __builtin_fabs(x); }
     static __device__ __forceinline__ double copysign(double a, double b) { return __builtin_copysign(a, b); }
     static __device__ __forceinline__ double fmax(double a, double b) { return __builtin_fmax(a, b); }
+    static __device__ __forceinline__ double floor(double x) { return __builtin_floor(x); }
 };
 
 // per-label constants in LDS, derived once per block from (mu_a, mu_s, g, n)
@@ -65,7 +67,8 @@ template <typename R> struct LabelConst {
     R hg4;     // 2 g
     R hginv;   // 1 / (2 g)
     R ms;      // mu_s
-    R pad[2];
+    R inv_ma;  // 1 / mu_a, 0 where mu_a = 0 (r-z fluence weight)
+    R pad;
 };
 
 // ------------------------------------------------------------------ kernel parameters
@@ -108,6 +111,9 @@ struct WalkParams {
     const float* rings;
     double* ring_w;
     float* exit_state;
+    int32_t rz_nr, rz_nz;      // r-z internal fluence bins [i dr, (i+1) dr) x [k dz, (k+1) dz)
+    float rz_dr, rz_dz;
+    double* fluence_rz;
     unsigned long long* work;
     const uint32_t* key_dev;   // (seed, stream) in device memory, or null
     // LDS-privatised box of the absorbed tally around the launch voxel
@@ -115,7 +121,7 @@ struct WalkParams {
     uint32_t win_nx, win_ny, win_nz, win_cells;
     // dynamic LDS layout (bytes)
     uint32_t lds_tab, lds_scal, lds_ev, lds_hist, lds_win;
-    uint32_t h_rd, h_tt, h_top, h_rda, h_tta, h_line, h_ring, h_total;  // offsets in doubles
+    uint32_t h_rd, h_tt, h_top, h_rda, h_tta, h_line, h_ring, h_rz, h_total;  // offsets in doubles
 };
 
 // block-level accumulator slots in LDS (doubles)
@@ -173,7 +179,8 @@ __device__ __forceinline__ LabelConst<R> label_const(const float4 o)
     c.hg4 = (R)2 * g;
     c.hginv = (g != (R)0) ? (R)1 / ((R)2 * g) : (R)0;
     c.ms = ms;
-    c.pad[0] = c.pad[1] = (R)0;
+    c.inv_ma = (ma > (R)0) ? (R)1 / ma : (R)0;
+    c.pad = (R)0;
     return c;
 }
 
@@ -284,7 +291,9 @@ __device__ __forceinline__ void launch_xy(int kind, float radius, R pitch, R hal
 }
 
 // ------------------------------------------------------------------ the walk kernel
-template <typename R>
+// RZ: also tally the r-z fluence (a separate instantiation, so the plain walk keeps its
+// register budget)
+template <typename R, bool RZ>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) walk_kernel(const WalkParams P)
 {
     using M = Mth<R>;
@@ -475,6 +484,17 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
                 const double dep = (double)w - (double)wn;        // exact in FP64
                 w = wn;
                 acc_abs += dep;
+                if (RZ && L.inv_ma > (R)0) {
+                    // r-z fluence: deposit / mu_a binned at the interaction point, in the
+                    // encoded coordinates of voxel/_classes.py:261-269
+                    const R x = (R)P.pitch * (R)(ax - P.cx) + px;
+                    const R y = (R)P.pitch * (R)(ay - P.cy) + py;
+                    const R z = (R)P.pitch * (R)(az - 1) + pz + half;
+                    const int ir = (int)M::floor(M::sqrt(x * x + y * y) / (R)P.rz_dr);
+                    const int iz = (int)M::floor(z / (R)P.rz_dz);
+                    if ((uint32_t)ir < (uint32_t)P.rz_nr && (uint32_t)iz < (uint32_t)P.rz_nz)
+                        lds_add(&hist[P.h_rz + (uint32_t)ir * (uint32_t)P.rz_nz + (uint32_t)iz], dep * (double)L.inv_ma);
+                }
                 if (P.absorbed) {
                     const uint32_t lx = (uint32_t)(ax - P.win_x0), ly = (uint32_t)(ay - P.win_y0),
                                    lz = (uint32_t)(az - P.win_z0);
@@ -604,7 +624,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
         else if (i < P.h_tta) dst = P.rd_a + (i - P.h_rda);
         else if (i < P.h_line) dst = P.tt_a + (i - P.h_tta);
         else if (i < P.h_ring) dst = P.line_z + (i - P.h_line);
-        else dst = P.ring_w + (i - P.h_ring);
+        else if (i < P.h_rz) dst = P.ring_w + (i - P.h_ring);
+        else dst = P.fluence_rz + (i - P.h_rz);
         gmem_add(dst, v);
     }
     for (uint32_t i = threadIdx.x; i < P.win_cells; i += blockDim.x) {
@@ -614,6 +635,30 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
         const uint32_t ly = t % P.win_ny, lx = t / P.win_ny;
         const uint32_t g = (((uint32_t)P.win_x0 + lx) * P.ny + (uint32_t)P.win_y0 + ly) * P.nz + (uint32_t)P.win_z0 + lz;
         gmem_add(P.absorbed + g, v);
+    }
+}
+
+// Fluence2D / Fluence3D binning of points (pymopt/fluence/intarnal_fluence.py)
+__global__ void __launch_bounds__(256) fluence_bin_kernel(const float* p, unsigned long long stride, const float* w,
+                                                          unsigned long long n, int dims, int nr, float dr, int nz,
+                                                          float dz, double* out)
+{
+    const unsigned long long step = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long j = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += step) {
+        const float x = p[j], y = p[stride + j], z = p[2 * stride + j];
+        const int iz = (int)__builtin_floorf(z / dz);
+        if (!(iz >= 0 && iz < nz)) continue;
+        unsigned long long b;
+        if (dims == 2) {
+            const int ir = (int)__builtin_floorf(__builtin_sqrtf(x * x + y * y) / dr);
+            if (!(ir >= 0 && ir < nr)) continue;
+            b = (unsigned long long)ir * nz + iz;
+        } else {
+            const int ix = (int)__builtin_floorf(x / dr) + nr, iy = (int)__builtin_floorf(y / dr) + nr;
+            if (!(ix >= 0 && ix < 2 * nr && iy >= 0 && iy < 2 * nr)) continue;
+            b = ((unsigned long long)ix * (2 * nr) + iy) * nz + iz;
+        }
+        gmem_add(out + b, (double)w[j]);
     }
 }
 

@@ -34,6 +34,7 @@ class TallySpec:
     absorbed: bool = True
     exit_state: bool = False
     events: bool = True
+    rz: tuple = ()               # (nr, dr, nz, dz): r-z internal fluence bins, mm
 
 
 class TallySet:
@@ -59,6 +60,8 @@ class TallySet:
             r = np.asarray(spec.rings, np.float32).reshape(-1)
             self.rings = torch.from_numpy(r).to(device)
             self.hist["ring_w"] = torch.zeros(r.size // 2, **f64)
+        if spec.rz:
+            self.hist["fluence_rz"] = torch.zeros(int(spec.rz[0]) * int(spec.rz[2]), **f64)
         self.exit_state = (torch.zeros(7 * int(n_packets), dtype=torch.float32, device=device)
                            if spec.exit_state else None)
         self.n_exit = int(n_packets) if spec.exit_state else 0
@@ -90,6 +93,9 @@ class TallySet:
         t.rings = _ptr(self.rings)
         t.ring_w = _ptr(self.hist.get("ring_w"))
         t.exit_state = _ptr(self.exit_state)
+        if s.rz:
+            t.rz_nr, t.rz_dr, t.rz_nz, t.rz_dz = int(s.rz[0]), float(s.rz[1]), int(s.rz[2]), float(s.rz[3])
+            t.fluence_rz = _ptr(self.hist["fluence_rz"])
         return t
 
     def to_host(self, shape=None) -> dict:
@@ -102,6 +108,8 @@ class TallySet:
             out["absorbed"] = a.reshape(shape) if shape is not None else a
         for k, v in self.hist.items():
             out[k] = v.cpu().numpy()
+        if self.spec.rz:
+            out["fluence_rz"] = out["fluence_rz"].reshape(int(self.spec.rz[0]), int(self.spec.rz[2]))
         if self.exit_state is not None:
             out["exit_state"] = self.exit_state.cpu().numpy().reshape(7, self.n_exit)
         return out

@@ -192,6 +192,15 @@ def normalise(raw: dict, n_packets: int, spec: TallySpec, pitch: float, optics: 
         area = np.pi * (rings[:, 1] ** 2 - rings[:, 0] ** 2)
         out["rings"] = rings
         out["ring_R"] = raw["ring_w"] / (area * N)
+    if spec.rz and "fluence_rz" in raw:
+        # Fluence2D's bins (pymopt/fluence/intarnal_fluence.py:92-106): annulus volume
+        # pi ((i+1)^2 - i^2) dr^2 dz, centres (i + 1/2) dr and (k + 1/2) dz
+        rz_nr, rz_dr, rz_nz, rz_dz = spec.rz
+        i = np.arange(int(rz_nr))
+        vol = np.pi * ((i + 1) ** 2 - i ** 2) * rz_dr ** 2 * rz_dz
+        out["fluence_rz"] = raw["fluence_rz"] / (vol[:, None] * N)
+        out["rz_r"] = (i + 0.5) * rz_dr
+        out["rz_z"] = (np.arange(int(rz_nz)) + 0.5) * rz_dz
     if "absorbed" in raw:
         a = raw["absorbed"]
         if shape is not None:
@@ -211,7 +220,7 @@ class BaseVoxelMonteCarlo:
     def __init__(self, *, nPh, model, dtype_f=np.float32, dtype=np.int32, beam_type="TEM00", w_beam=0,
                  beam_angle=0, initial_refrect_by_angle=False, first_layer_clear=False, threadnum=128,
                  nr=50, dr=0.1, na=0, nl=0, dl=0.1, line_y=5.0, rings=(), fluence=True,
-                 exit_state=False, fp64=False, normal_fresnel=False, device="cuda:0"):
+                 exit_state=False, fluence_rz=None, fp64=False, normal_fresnel=False, device="cuda:0"):
         beam_type_list = ["TEM00", "flat", False]
         if beam_type not in beam_type_list:
             raise ValueError("%s is not a permitted for factor. Please choose from %s." % (beam_type, beam_type_list))
@@ -233,7 +242,8 @@ class BaseVoxelMonteCarlo:
         self.threadnum = int(threadnum)
         self.spec = TallySpec(nr=int(nr), dr=float(dr), na=int(na), nl=int(nl), dl=float(dl),
                               line_y=float(line_y), rings=tuple(tuple(r) for r in rings),
-                              absorbed=bool(fluence), exit_state=bool(exit_state))
+                              absorbed=bool(fluence), exit_state=bool(exit_state),
+                              rz=tuple(fluence_rz) if fluence_rz else ())
         self.fp64 = bool(fp64)
         self.normal_fresnel = bool(normal_fresnel)
         self.device = device

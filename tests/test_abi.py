@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), n
         assert n in _abi.SIGNATURES, n
-    assert L.pmo_abi_version() == _abi.ABI_VERSION == 2
+    assert L.pmo_abi_version() == _abi.ABI_VERSION == 3
     assert L.pmo_workspace_bytes() >= 8
 
 
@@ -40,7 +40,7 @@ int main(void) {
   S(pmo_volume) S(pmo_source) S(pmo_run) S(pmo_tallies) S(pmo_plate_instance) S(pmo_batch_tallies) S(pmo_diff) S(pmo_obd)
   O(pmo_diff, mismatch) O(pmo_plate_instance, n_packets) O(pmo_plate_instance, layers) O(pmo_run, key_dev) O(pmo_run, quorum)
   O(pmo_volume, optics) O(pmo_source, w0) O(pmo_run, packets_per_lane) O(pmo_tallies, exit_state)
-  O(pmo_tallies, ring_w) O(pmo_tallies, line_z) O(pmo_run, flags) O(pmo_run, window)
+  O(pmo_tallies, ring_w) O(pmo_tallies, line_z) O(pmo_tallies, fluence_rz) O(pmo_run, flags) O(pmo_run, window)
   O(pmo_obd, stream) O(pmo_obd, side)
   printf("PMO_S_COUNT %d\nPMO_EV_COUNT %d\nPMO_OBD_STEP_FIELDS %d\n", PMO_S_COUNT, PMO_EV_COUNT, PMO_OBD_STEP_FIELDS);
   return 0;
@@ -73,6 +73,7 @@ def test_ctypes_layout_matches_header(tmp_path):
     assert int(out["pmo_tallies.exit_state"]) == _abi.PmoTallies.exit_state.offset
     assert int(out["pmo_tallies.ring_w"]) == _abi.PmoTallies.ring_w.offset
     assert int(out["pmo_tallies.line_z"]) == _abi.PmoTallies.line_z.offset
+    assert int(out["pmo_tallies.fluence_rz"]) == _abi.PmoTallies.fluence_rz.offset
     assert int(out["PMO_S_COUNT"]) == _abi.PMO_S_COUNT
     assert int(out["pmo_obd.stream"]) == _abi.PmoObd.stream.offset
     assert int(out["pmo_obd.side"]) == _abi.PmoObd.side.offset

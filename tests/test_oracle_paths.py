@@ -124,3 +124,58 @@ def test_path_sums_match_the_event_trace(oracle):
         else:
             assert wl.sum() == 0 and wn.sum() == 0
     assert checked > 50
+
+
+def test_rz_fluence_non_scattering_closed_form(oracle):
+    # each packet drops all of w0 at one depth z ~ Exp(mu_a) on the axis; the r-z tally holds
+    # w0/mu_a per packet in the bin of that depth. dz = 0.03 does not align with the 0.05 mm
+    # voxels, so the tally must bin interaction positions, not voxels
+    ma, d, l, n = 2.0, 0.5, 0.05, 1.4
+    vol = plate_volume([d], l, 1.05, 1.05)
+    opt = optics_table([ma], [0.0], [0.0], [n], 1.0)
+    cx, cy = center_add(vol.shape[0]), center_add(vol.shape[1])
+    w0 = 1 - specular(1.0, n)
+    N, dz, nz = 40000, 0.03, 16
+    out = oracle.walk(vol, l, *_cols(opt), dict(kind=0, voxel=(cx, cy, 1), w0=w0), N, 6, absorbed=False,
+                      rz=(3, 0.1, nz, dz))
+    F = out["fluence_rz"]
+    assert F.shape == (3, nz) and F[1:].sum() == 0.0
+    k = np.arange(nz)
+    p = np.exp(-ma * k * dz) - np.exp(-ma * (k + 1) * dz)
+    got = F[0] * ma / (w0 * N)
+    assert np.all(np.abs(got - p) < 4 * np.sqrt(p * (1 - p) / N))
+
+
+def test_rz_fluence_radial_closed_form_flat_beam(oracle):
+    # a flat disc of radius R puts the (single) interaction of each packet at r with
+    # P(r in [i dr, (i+1) dr)) = ((i+1)^2 - i^2) dr^2 / R^2
+    ma, d, l, n = 1.0, 0.3, 0.05, 1.0
+    vol = plate_volume([d], l, 2.05, 2.05)
+    opt = optics_table([ma], [0.0], [0.0], [n], 1.0)
+    cx, cy = center_add(vol.shape[0]), center_add(vol.shape[1])
+    R, dr, nr, N = 0.6, 0.07, 10, 40000
+    out = oracle.walk(vol, l, *_cols(opt), dict(kind=1, voxel=(cx, cy, 1), radius=R, w0=1.0), N, 8,
+                      absorbed=False, rz=(nr, dr, 1, 1.0))
+    F = out["fluence_rz"][:, 0] * ma                  # w0 = 1: one unit per interacting packet
+    n_int = F.sum()
+    frac = F / n_int
+    i = np.arange(nr)
+    p = np.clip(((i + 1) ** 2 - i ** 2) * dr ** 2 / R ** 2, 0, None)
+    p[i * dr >= R] = 0.0
+    last = int(R // dr)
+    p[last] = 1 - p[:last].sum()                      # the bin holding the disc edge
+    assert n_int == pytest.approx(N * (1 - np.exp(-ma * d)), rel=0.05)
+    assert np.all(np.abs(frac - p) < 4 * np.sqrt(p * (1 - p) / n_int) + 1e-12)
+
+
+def test_rz_fluence_times_mua_sums_to_the_absorbed_weight(oracle):
+    # one label: every deposit lands in some bin when the bins cover the slab, so
+    # mu_a * sum(fluence) equals the absorbed scalar A
+    ma, d, l = 0.3, 1.0, 0.05
+    vol = plate_volume([d], l, 2.05, 2.05)
+    opt = optics_table([ma], [8.0], [0.8], [1.4], 1.0)
+    cx, cy = center_add(vol.shape[0]), center_add(vol.shape[1])
+    out = oracle.walk(vol, l, *_cols(opt), dict(kind=0, voxel=(cx, cy, 1), w0=0.97), 3000, 12, absorbed=False,
+                      rz=(30, 0.05, 26, 0.04))
+    # the table holds mu_a in float32, and the tally divides by that value
+    assert out["fluence_rz"].sum() * float(opt[0, 0]) == pytest.approx(out["scalars"]["A"], rel=1e-12)
