@@ -191,6 +191,34 @@ def cpu_baseline_obd(params, host_es, sample):
             "sample": f"{wt.size} exits x {Zs.size} focus steps, numpy float64, {tc:.1f} s"}
 
 
+def turing_leg():
+    """The Turing-pattern bone model at the reference's defaults: 520^3 grid, 100 steps
+    (reaction kernel, 2 R2C + 2 C2R hipFFT transforms, spectrum kernel per step)."""
+    from pymonteopt_amd.modeling import TuringPattern
+    t = TuringPattern()
+    t.set_params(seed=1)
+    u, v = t.grow(steps=2)                 # warm-up: plans, code objects
+    del u, v
+    u, v = t.grow()
+    steps, n = int(t.repetition), t.gridsize
+    bv, peaks = t.microarchitecture_of(u)
+    return {"grid": n, "steps": steps, "seconds": t.time_s, "ms_per_step": 1e3 * t.time_s / steps,
+            "cell_steps_per_s": n ** 3 * steps / t.time_s, "bv_tv_real": bv, "peaks": peaks}
+
+
+def cpu_baseline_turing(n=96, steps=3):
+    """The oracle's numpy restatement (float64 FFTs, one process) on an n^3 grid."""
+    from oracle import turing as tp
+    u, v = (x.astype("float64") for x in tp.initial_fields(n, 1))
+    ku, kv = tp.inverse_symbol(n, 0.0002, 1.0, 1 / 40), tp.inverse_symbol(n, 0.01, 1.0, 1 / 40)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        u, v = tp.step(u, v, ku, kv, 1.0)
+    dt = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n}^3 grid x {steps} steps, numpy float64 FFTs, {dt:.1f} s"}
+
+
 def measure_peaks():
     """Re-measures the two peaks with libpmo's probes: FP64 global atomic adds per second
     (pseudo-random addresses in 32 MiB) and FP32 FMA lane-ops per second."""
@@ -383,11 +411,13 @@ def main():
         out["batched"] = batched_leg()
         out["adjoint"] = adjoint_leg()
         out["obd"], obd_sample = obd_leg()
+        out["turing"] = turing_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl)
         if not args.no_extra:
             out["obd"]["cpu_baseline"] = cpu_baseline_obd(*obd_sample)
             out["obd"]["gpu_vs_cpu"] = out["obd"]["traces_per_s"] / out["obd"]["cpu_baseline"]["value"]
+            out["turing"]["cpu_baseline"] = cpu_baseline_turing()
     print(json.dumps(out), flush=True)
 
 

@@ -308,6 +308,36 @@ int pmo_obd_trace(const pmo_obd* cfg, const float* p, const float* v, const floa
 int pmo_fluence_bin(const float* p, int64_t stride, const float* w, int64_t n, int32_t dims, int32_t nr, float dr,
                     int32_t nz, float dz, double* out, void* stream);
 
+/* ---- Turing-pattern bone model ------------------------------------------------------ */
+/* TuringPattern of pymopt/modeling_gpu/_classes.py:25-176: a reaction-diffusion system on
+ * a periodic n^3 grid whose thresholded activator is the trabecular-bone label volume. */
+typedef struct pmo_turing pmo_turing;   /* the FFT plans of one grid size (opaque)             */
+typedef struct pmo_turing_params {
+    float du, dv;          /* diffusion coefficients of the activator u and inhibitor v          */
+    float dx, dt;          /* grid spacing and time step                                         */
+    float a1, a2, a3;      /* f(u, v) = u + dt (a1 u + a2 v + a3 u^3); 0.6, -1, -1 (:51)          */
+    float b1, b2;          /* g(u, v) = v + dt (b1 u + b2 v); 1.5, -2 (:52)                       */
+} pmo_turing_params;
+
+/* Plans the R2C and C2R transforms of an n^3 grid (2 <= n <= 1024). hipFFT keeps its own
+ * small tables; the transforms' work area and the two spectra come from the caller's
+ * workspace of pmo_turing_workspace_bytes(). */
+int pmo_turing_create(int32_t n, pmo_turing** out);
+int64_t pmo_turing_workspace_bytes(const pmo_turing* h);
+int pmo_turing_destroy(pmo_turing* h);
+/* `steps` steps of _calc_onestep (:112-115) on device fields u, v [n][n][n] in place:
+ * u <- Re F^-1[Re F[f(u, v)] / lambda_u], v <- Re F^-1[Re F[g(u, v)] / lambda_v], lambda
+ * the symbol of the implicit 7-point diffusion stencil (:83-104). */
+int pmo_turing_step(pmo_turing* h, const pmo_turing_params* prm, float* u, float* v, void* workspace,
+                    int32_t steps, void* stream);
+/* u <= thr -> 0, adding the number of such cells to *n_below (device) (:134-138). */
+int pmo_threshold_count(float* u, int64_t n, float thr, uint64_t* n_below, void* stream);
+/* Adds to *count (device) the strict local maxima of u along rows of length len against
+ * `order` neighbours each side, edges clipped (scipy argrelmax; :140-160). */
+int pmo_count_peaks(const float* u, int64_t rows, int32_t len, int32_t order, uint64_t* count, void* stream);
+/* labels = u > 0 (:162-165). */
+int pmo_binarize(const float* u, int64_t n, int8_t* labels, void* stream);
+
 /* Many plates in one persistent launch. Packet j of instance i draws from Philox key
  * (seed, stream + i) with packet id j, i.e. exactly the draws of a plain pmo_walk of that
  * plate with stream + i, so the sweep reproduces the sequential runs it replaces: one

@@ -91,6 +91,11 @@ class PmoObd(C.Structure):
                 ("stream", C.c_uint32)]
 
 
+class PmoTuringParams(C.Structure):
+    _fields_ = [("du", C.c_float), ("dv", C.c_float), ("dx", C.c_float), ("dt", C.c_float), ("a1", C.c_float),
+                ("a2", C.c_float), ("a3", C.c_float), ("b1", C.c_float), ("b2", C.c_float)]
+
+
 class PmoDiff(C.Structure):
     _fields_ = [("mua_field", C.c_void_p), ("nr", C.c_int32), ("dr", C.c_float), ("rd_r", C.c_void_p),
                 ("jac", C.c_void_p), ("exit_rec", C.c_void_p), ("coef", C.c_void_p), ("grad_mua", C.c_void_p),
@@ -110,6 +115,13 @@ SIGNATURES = {
                                C.c_void_p, C.POINTER(PmoDiff), C.c_void_p, C.c_void_p], C.c_int),
     "pmo_walk_adjoint": ([C.POINTER(PmoVolume), C.POINTER(PmoSource), C.POINTER(PmoRun), C.POINTER(PmoDiff),
                           C.c_void_p, C.c_void_p], C.c_int),
+    "pmo_turing_create": ([C.c_int32, C.c_void_p], C.c_int),
+    "pmo_turing_workspace_bytes": ([C.c_void_p], C.c_int64),
+    "pmo_turing_destroy": ([C.c_void_p], C.c_int),
+    "pmo_turing_step": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
+    "pmo_threshold_count": ([C.c_void_p, C.c_int64, C.c_float, C.c_void_p, C.c_void_p], C.c_int),
+    "pmo_count_peaks": ([C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p], C.c_int),
+    "pmo_binarize": ([C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p], C.c_int),
     "pmo_fluence_bin": ([C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_int32,
                          C.c_float, C.c_void_p, C.c_void_p], C.c_int),
     "pmo_obd_trace": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int32,

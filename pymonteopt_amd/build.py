@@ -32,6 +32,9 @@ FLAGS = [
     "-Wno-unused-function",
     f"-I{INCLUDE}",
     f"-I{CSRC}",
+    "-L/opt/rocm/lib",
+    "-lhipfft",                     # the Turing model's 3-D transforms
+    "-Wl,-rpath,/opt/rocm/lib",
 ]
 
 

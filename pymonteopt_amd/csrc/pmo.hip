@@ -15,6 +15,7 @@
 #include "adjoint.hpp"
 #include "batch.hpp"
 #include "obd.hpp"
+#include "turing.hpp"
 #include "walk.hpp"
 
 namespace {
@@ -565,6 +566,159 @@ int pmo_walk_plates(const pmo_plate_instance* inst, int32_t n_inst, const uint32
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "batched walk launch");
     return PMO_OK;
+}
+
+}  // extern "C"
+
+struct pmo_turing {
+    int n;
+    hipfftHandle r2c, c2r;
+    size_t work_bytes, spec_bytes;
+};
+
+namespace {
+
+unsigned grid_for(unsigned long long n, int n_cu)
+{
+    unsigned long long b = (n + 255) / 256;
+    const unsigned long long cap = (unsigned long long)n_cu * 16ull;
+    return (unsigned)(b < 1 ? 1 : (b > cap ? cap : b));
+}
+
+int fft_fail(hipfftResult r, const char* what) { return fail(PMO_ERR_HIP, "%s: hipFFT error %d", what, (int)r); }
+
+}  // namespace
+
+extern "C" {
+
+int pmo_turing_create(int32_t n, pmo_turing** out)
+{
+    g_err[0] = '\0';
+    if (!out) return fail(PMO_ERR_INVALID, "null output");
+    *out = nullptr;
+    if (n < 2 || n > 1024) return fail(PMO_ERR_INVALID, "grid size %d outside [2, 1024]", n);
+    pmo_turing* h = new pmo_turing();
+    h->n = n;
+    size_t w1 = 0, w2 = 0;
+    hipfftResult r = hipfftCreate(&h->r2c);
+    if (r == HIPFFT_SUCCESS) r = hipfftCreate(&h->c2r);
+    if (r == HIPFFT_SUCCESS) r = hipfftSetAutoAllocation(h->r2c, 0);
+    if (r == HIPFFT_SUCCESS) r = hipfftSetAutoAllocation(h->c2r, 0);
+    if (r == HIPFFT_SUCCESS) r = hipfftMakePlan3d(h->r2c, n, n, n, HIPFFT_R2C, &w1);
+    if (r == HIPFFT_SUCCESS) r = hipfftMakePlan3d(h->c2r, n, n, n, HIPFFT_C2R, &w2);
+    if (r != HIPFFT_SUCCESS) {
+        pmo_turing_destroy(h);
+        return fft_fail(r, "plan");
+    }
+    h->work_bytes = align_up(w1 > w2 ? w1 : w2, 256);
+    h->spec_bytes = align_up((size_t)n * n * (n / 2 + 1) * sizeof(float2), 256);
+    *out = h;
+    return PMO_OK;
+}
+
+int64_t pmo_turing_workspace_bytes(const pmo_turing* h)
+{
+    if (!h) return fail(PMO_ERR_INVALID, "null handle");
+    return (int64_t)(2 * h->spec_bytes + h->work_bytes);
+}
+
+int pmo_turing_destroy(pmo_turing* h)
+{
+    if (!h) return PMO_OK;
+    if (h->r2c) hipfftDestroy(h->r2c);
+    if (h->c2r) hipfftDestroy(h->c2r);
+    delete h;
+    return PMO_OK;
+}
+
+int pmo_turing_step(pmo_turing* h, const pmo_turing_params* prm, float* u, float* v, void* workspace,
+                    int32_t steps, void* stream)
+{
+    g_err[0] = '\0';
+    if (!h || !prm || !u || !v || !workspace) return fail(PMO_ERR_INVALID, "null argument");
+    if (steps < 0) return fail(PMO_ERR_INVALID, "steps < 0");
+    if (!(prm->dx > 0.f) || !(prm->dt > 0.f) || prm->du < 0.f || prm->dv < 0.f)
+        return fail(PMO_ERR_INVALID, "need dx, dt > 0 and du, dv >= 0");
+    DevInfo d;
+    int rc = device_info(&d);
+    if (rc != PMO_OK) return rc;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int n = h->n;
+    const unsigned long long cells = (unsigned long long)n * n * n;
+    float2* U = reinterpret_cast<float2*>(workspace);
+    float2* V = reinterpret_cast<float2*>(reinterpret_cast<char*>(workspace) + h->spec_bytes);
+    void* work = reinterpret_cast<char*>(workspace) + 2 * h->spec_bytes;
+    hipfftResult r = hipfftSetStream(h->r2c, s);
+    if (r == HIPFFT_SUCCESS) r = hipfftSetStream(h->c2r, s);
+    if (r == HIPFFT_SUCCESS) r = hipfftSetWorkArea(h->r2c, work);
+    if (r == HIPFFT_SUCCESS) r = hipfftSetWorkArea(h->c2r, work);
+    if (r != HIPFFT_SUCCESS) return fft_fail(r, "stream / work area");
+    const pmo::TuringCoef c{prm->dt, prm->a1, prm->a2, prm->a3, prm->b1, prm->b2};
+    const float cu = prm->dt * prm->du / (prm->dx * prm->dx), cv = prm->dt * prm->dv / (prm->dx * prm->dx);
+    const float scale = (float)(1.0 / (double)cells);
+    const unsigned gr = grid_for(cells, d.n_cu);
+    const unsigned gs = grid_for((unsigned long long)n * n * (n / 2 + 1), d.n_cu);
+    for (int it = 0; it < steps; ++it) {
+        hipLaunchKernelGGL(pmo::turing_reaction_kernel, dim3(gr), dim3(256), 0, s, u, v, cells, c);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "reaction launch");
+        r = hipfftExecR2C(h->r2c, u, reinterpret_cast<hipfftComplex*>(U));
+        if (r == HIPFFT_SUCCESS) r = hipfftExecR2C(h->r2c, v, reinterpret_cast<hipfftComplex*>(V));
+        if (r != HIPFFT_SUCCESS) return fft_fail(r, "forward transform");
+        hipLaunchKernelGGL(pmo::turing_spectrum_kernel, dim3(gs), dim3(256), (size_t)n * sizeof(float), s, U, V, n,
+                           cu, cv, scale);
+        e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "spectrum launch");
+        r = hipfftExecC2R(h->c2r, reinterpret_cast<hipfftComplex*>(U), u);
+        if (r == HIPFFT_SUCCESS) r = hipfftExecC2R(h->c2r, reinterpret_cast<hipfftComplex*>(V), v);
+        if (r != HIPFFT_SUCCESS) return fft_fail(r, "inverse transform");
+    }
+    return PMO_OK;
+}
+
+int pmo_threshold_count(float* u, int64_t n, float thr, uint64_t* n_below, void* stream)
+{
+    g_err[0] = '\0';
+    if (!u || !n_below || n < 0) return fail(PMO_ERR_INVALID, "bad argument");
+    if (n == 0) return PMO_OK;
+    DevInfo d;
+    int rc = device_info(&d);
+    if (rc != PMO_OK) return rc;
+    hipLaunchKernelGGL(pmo::threshold_kernel, dim3(grid_for((unsigned long long)n, d.n_cu)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), u, (unsigned long long)n, thr,
+                       reinterpret_cast<unsigned long long*>(n_below));
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PMO_OK : hip_fail(e, "threshold launch");
+}
+
+int pmo_count_peaks(const float* u, int64_t rows, int32_t len, int32_t order, uint64_t* count, void* stream)
+{
+    g_err[0] = '\0';
+    if (!u || !count || rows < 0 || len < 1 || order < 1) return fail(PMO_ERR_INVALID, "bad argument");
+    if (rows == 0) return PMO_OK;
+    DevInfo d;
+    int rc = device_info(&d);
+    if (rc != PMO_OK) return rc;
+    const unsigned long long total = (unsigned long long)rows * (unsigned long long)len;
+    hipLaunchKernelGGL(pmo::count_peaks_kernel, dim3(grid_for(total, d.n_cu)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), u, (unsigned long long)rows, len, order,
+                       reinterpret_cast<unsigned long long*>(count));
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PMO_OK : hip_fail(e, "peak count launch");
+}
+
+int pmo_binarize(const float* u, int64_t n, int8_t* labels, void* stream)
+{
+    g_err[0] = '\0';
+    if (!u || !labels || n < 0) return fail(PMO_ERR_INVALID, "bad argument");
+    if (n == 0) return PMO_OK;
+    DevInfo d;
+    int rc = device_info(&d);
+    if (rc != PMO_OK) return rc;
+    hipLaunchKernelGGL(pmo::binarize_kernel, dim3(grid_for((unsigned long long)n, d.n_cu)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), u, (unsigned long long)n, labels);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PMO_OK : hip_fail(e, "binarize launch");
 }
 
 int pmo_fluence_bin(const float* p, int64_t stride, const float* w, int64_t n, int32_t dims, int32_t nr, float dr,
