@@ -24,7 +24,7 @@ from .geometry import (Source, center_add, fresnel_oblique, optics_table, plate_
 from .runtime import TallySpec, Walker
 
 __all__ = ["set_params", "PlateModel", "LabelVolumeModel", "BaseVoxelMonteCarlo", "VoxelPlateModel",
-           "VoxelLabelModel", "normalise"]
+           "VoxelLabelModel", "VoxelTuringModel", "normalise"]
 
 
 def set_params(data, keys, *initial_data, **kwargs):
@@ -279,7 +279,7 @@ class BaseVoxelMonteCarlo:
         """Launch description, pymopt/voxel/_classes.py:107-213."""
         vm = self.model.voxel_model
         ix, iy = center_add(vm.shape[0]), center_add(vm.shape[1])
-        iz = int(getattr(self.model, "params", {}).get("launch_z", 1))
+        iz = int(getattr(self.model, "launch_z", getattr(self.model, "params", {}).get("launch_z", 1)))
         n1, n2 = self.model.n_ambient, self.model.n_first
         direction = (0.0, 0.0, 1.0)
         rsp = specular(n1, n2)
@@ -414,5 +414,40 @@ class VoxelLabelModel(BaseVoxelMonteCarlo):
         if initial_data or kwargs:
             self.model.set_params(*initial_data, **kwargs)
         self.model.build(volume)
+        self._walker = None
+        return self
+
+
+class VoxelTuringModel(BaseVoxelMonteCarlo):
+    """pymopt/voxel/_classes.py:847-1000: a walk through a Turing-pattern bone volume
+    wrapped by TuringModel_Rectangular, _Cylinder or _RnC. `set_model(u)` takes the 0/1
+    volume (TuringPattern.modeling's output), `build(**params)` wraps it. The reference's
+    default name 'TuringModel' matches none of its classes (it fails there); here it
+    selects the rectangular model."""
+
+    def __init__(self, *, nPh=1000, beam_type="TEM00", w_beam=0, beam_angle=0, threadnum=128,
+                 model_name="TuringModel_Rectangular", **tally_kwargs):
+        from .bone_models import TuringModel_Cylinder, TuringModel_Rectangular, TuringModel_RnC
+        models = {"TuringModel_Rectangular": TuringModel_Rectangular, "TuringModel": TuringModel_Rectangular,
+                  "TuringModel_Cylinder": TuringModel_Cylinder, "TuringModel_RnC": TuringModel_RnC}
+        if model_name not in models:
+            raise ValueError("Invalid name: %s (choose from %s)" % (model_name, sorted(models)))
+        super().__init__(nPh=nPh, model=models[model_name](), beam_type=beam_type, w_beam=w_beam,
+                         beam_angle=beam_angle, threadnum=threadnum, **tally_kwargs)
+        self.bone_model = None
+
+    def set_model(self, u):
+        self.bone_model = u
+
+    def set_params(self, *initial_data, **kwargs):
+        self.model.set_params(*initial_data, **kwargs)
+
+    def build(self, *initial_data, **kwargs):
+        if initial_data or kwargs:
+            self.model.set_params(*initial_data, **kwargs)
+        if self.bone_model is None:
+            raise ValueError("set_model(u) first")
+        self.model.build(self.bone_model)
+        self.bone_model = None
         self._walker = None
         return self

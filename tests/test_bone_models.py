@@ -1,0 +1,86 @@
+"""The layered bone geometries of TuringModel_Rectangular / _Cylinder / _RnC against layer
+positions and radii derived by hand from their parameters (no GPU)."""
+import numpy as np
+import pytest
+
+from pymonteopt_amd.bone_models import TuringModel_Cylinder, TuringModel_Rectangular, TuringModel_RnC
+
+
+def bone(n=40, seed=0):
+    return (np.random.default_rng(seed).uniform(size=(n, n, n)) < 0.3).astype(np.int8)
+
+
+def test_rectangular_layers():
+    m = TuringModel_Rectangular()
+    m.set_params(voxel_space=0.1, xz_size=3.0, th_cortical=0.2, th_subcutaneus=0.3, th_dermis=0.4)
+    u = bone()
+    m.build(u)
+    vm = m.voxel_model
+    # layers 2, 3, 4 voxels; crop c = int(20 - round(15)) + 2 = 7 -> 26 in x and z; + 2 (2+3+4) + 2
+    assert vm.shape == (46, 42, 46) and m.end_point == 5 and m.launch_z == 1
+    assert np.all(vm[0] == 5) and np.all(vm[:, 0] == 5) and np.all(vm[:, :, -1] == 5)
+    inner = vm[1:-1, 1:-1, 1:-1]                       # 44 x 40 x 44
+    expect = np.full((44, 40, 44), 0, np.uint8)
+    for lab, lo in ((4, 0), (3, 4), (2, 7)):
+        expect[lo:44 - lo, :, lo:44 - lo] = lab
+    expect[9:35, :, 9:35] = u[7:-7, :, 7:-7]
+    assert np.array_equal(inner, expect)
+    assert m.optics().shape == (6, 4) and m.n_first == pytest.approx(1.4)
+    with pytest.raises(ValueError):
+        m.set_params(xz_size=10.0)
+        m.build(u)
+
+
+def _expected_quadrant(lo, radii, labels, core):
+    """Labels of the cross-section quadrant i, j <= 0 (relative to the centre). Each shell t
+    is the disc r < radii[t] inside its iteration's window i, j >= lo[t], less what earlier
+    ones hold; outside all of them is air. `core(i, j)` marks the bone region."""
+    n = -min(lo) + 1
+    out = np.full((n, n), -1)
+    for a in range(n):
+        for b in range(n):
+            i, j = a - (n - 1), b - (n - 1)
+            r = np.hypot(i, j)
+            if core(i, j):
+                out[a, b] = -2                      # bone volume (0/1)
+                continue
+            out[a, b] = 5
+            for t in range(len(radii)):
+                if i >= lo[t] and j >= lo[t] and r < radii[t]:
+                    out[a, b] = labels[t]
+                    break
+    return out
+
+
+@pytest.mark.parametrize("cls", [TuringModel_Cylinder, TuringModel_RnC])
+def test_cylinder_shells(cls):
+    m = cls()
+    kw = dict(voxel_space=0.1, r_bone=1.5, th_cortical=0.2, th_subcutaneus=0.3, th_dermis=0.3)
+    if cls is TuringModel_RnC:
+        kw["xz_size"] = 2.2
+    m.set_params(**kw)
+    u = bone()
+    m.build(u)
+    vm = m.voxel_model
+    assert m.end_point == 6 and m.optics().shape == (7, 4)
+    # core disc radius round(1.5/0.1) - 2 = 13 about voxel 20 of 40. Cylinder: first core
+    # row i = 8, crop 8 -> 24 cells, window from -12. RnC: rectangle from int(20 - 11) + 2 =
+    # 11 -> 18 cells, window from -9. Shells of 2, 3, 3 cells widen the window and the radius
+    if cls is TuringModel_Cylinder:
+        lo0, core = -12, (lambda i, j: i >= -12 and j >= -12 and np.hypot(i, j) < 13)
+    else:
+        lo0, core = -9, (lambda i, j: i >= -9 and j >= -9)
+    lo = [lo0 - 2, lo0 - 5, lo0 - 8]
+    want = _expected_quadrant(lo, [15, 18, 21], [2, 3, 4], core)
+    inner = vm[1:-1, 1:-1, 1:-1]
+    c0, c2 = inner.shape[0] // 2, inner.shape[2] // 2
+    n = want.shape[0]
+    got = inner[c0 - n + 1:c0 + 1, inner.shape[1] // 2, c2 - n + 1:c2 + 1].astype(int)
+    assert np.array_equal(np.where(want == -2, -2, got), want)
+    assert set(np.unique(got[want == -2])) <= {0, 1}
+    assert inner.shape[0] == 2 * (-lo[-1])               # the last window spans the volume
+    if cls is TuringModel_Cylinder:
+        col = vm[(vm.shape[0] - 1) // 2, (vm.shape[1] - 1) // 2]
+        assert col[m.launch_z] == 4 and np.all(col[1:m.launch_z] == 5)
+    else:
+        assert m.launch_z == 1

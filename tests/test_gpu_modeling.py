@@ -59,3 +59,35 @@ def test_modeling_end_to_end(tmp_path):
     assert info["model_shape"] == [n, n, n] and info["microarchitecture"]["BV/TV real"] == pytest.approx(t.bv_tv_real)
     with pytest.raises(NotImplementedError):
         t.modeling(str(tmp_path), save_dicom=True)
+
+
+@pytest.mark.parametrize("name", ["TuringModel_Rectangular", "TuringModel_Cylinder", "TuringModel_RnC"])
+def test_turing_volume_walk_matches_oracle(name, oracle):
+    from pymonteopt_amd.voxel import VoxelTuringModel
+    t = model(n_grid=8, length=5, reps=30)             # 40^3 bone volume
+    u, _ = t.grow()
+    t.microarchitecture_of(u)
+    bone = (u.cpu().numpy() > 0).astype(np.int8)
+    m = VoxelTuringModel(nPh=40000, model_name=name, nr=20, dr=0.1)
+    m.set_model(bone)
+    kw = dict(voxel_space=0.05, th_cortical=0.1, th_subcutaneus=0.15, th_dermis=0.1)
+    kw.update(dict(xz_size=1.4) if name != "TuringModel_Cylinder" else {})
+    kw.update(dict(r_bone=0.9) if name != "TuringModel_Rectangular" else {})
+    m.build(**kw)
+    vm = m.model.voxel_model
+    src = m.source()
+    assert vm[src.voxel] == (4 if name == "TuringModel_Cylinder" else vm[src.voxel])
+    rd = []
+    for s in range(6):
+        m.start(rand_seed=s)
+        r = m.get_result()
+        rd.append(r["Rd"])
+        assert abs(sum(r["energy"][k] for k in ("Rd", "Tt", "other", "A", "kill")) - r["energy"]["gain"]
+                   - r["energy"]["w0"]) < 1e-9
+    opt = m.model.optics().astype(np.float64)
+    L = opt.shape[0] - 1
+    ora = [oracle.walk(vm, 0.05, opt[:L, 0], opt[:L, 1], opt[:L, 2], opt[:, 3], src.as_dict(), 40000, 50 + s,
+                       absorbed=False)["scalars"]["Rd"] / 40000 for s in range(6)]
+    a, b = np.array(rd), np.array(ora)
+    se = np.sqrt(a.var(ddof=1) / a.size + b.var(ddof=1) / b.size)
+    assert abs(a.mean() - b.mean()) < 4 * se + 1e-6, (a.mean(), b.mean(), se)
