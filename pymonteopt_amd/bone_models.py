@@ -106,12 +106,17 @@ class _TuringBase:
             raise ValueError("xz_size and th_cortical leave nothing of the bone volume to crop")
         return c
 
-    def build(self, bone_model):
+    def build(self, bone_model=None):
         if self.params.get("dicom_path"):
-            raise NotImplementedError("DICOM input needs pydicom, which this build does not include")
-        vm = np.ascontiguousarray(bone_model).astype(np.uint8)
-        if vm.ndim != 3 or vm.max(initial=0) > 1:
+            # the stack of slice files replaces the volume, its spacing the voxel_space (:577-586)
+            from .dicom import read_stack
+            bone_model, vs = read_stack(self.params["dicom_path"])
+            self.params["voxel_space"] = vs
+            self.voxel_space = vs
+        vm = np.ascontiguousarray(bone_model)
+        if vm.ndim != 3 or vm.min(initial=0) < 0 or vm.max(initial=0) > 1:
             raise ValueError("the bone model must be a 3-D 0/1 volume")
+        vm = vm.astype(np.uint8)
         self.voxel_model = self._make_voxel_model(vm)
         self.getModelSize()
 

@@ -16,8 +16,8 @@ Differences:
 * Reaction terms are the reference's family f = u + dt (a1 u + a2 v + a3 u^3) and
   g = v + dt (b1 u + b2 v). `set_activator_function` and `set_inhibitor_function` accept a
   callable of that form and recover its coefficients; any other form raises ValueError.
-* DICOM output (`save_dicom=True`) needs pydicom, which is not part of this build, so it
-  raises NotImplementedError.
+* DICOM output (`save_dicom=True`) is written by `pymonteopt_amd.dicom`, this build's own
+  reader and writer of uncompressed little-endian files, because pydicom is absent.
 """
 from __future__ import annotations
 
@@ -153,11 +153,12 @@ class TuringPattern:
         return 1 - below / n, peaks
 
     def modeling(self, path, save_dicom=False):
-        if save_dicom:
-            raise NotImplementedError("DICOM output needs pydicom, which this build does not include")
+        self.save_dicom = save_dicom
         u, v = self.grow()
         del v
         self.model_shape = tuple(u.shape)
+        if save_dicom:
+            self._save_dicom(u.cpu().numpy(), path)
         print("Model Size: %s Mb" % (u.numel() * 4 / 1e6))
         self.bv_tv_real, peaks = self.microarchitecture_of(u)
         print("BV/TV = %.4f" % self.bv_tv_real)
@@ -171,6 +172,15 @@ class TuringPattern:
         if self.tile_num_xz != 0:
             U = np.tile(U, (self.tile_num_xz, self.tile_num_y, self.tile_num_xz))
         return U
+
+    def _save_dicom(self, field, path):
+        """The field times ct_coef as 16-bit slices along the last axis, path/0000.dcm ...
+        (TuringPattern._save_dicom, :178-183), written by pymonteopt_amd.dicom."""
+        from .dicom import write_slice
+        os.makedirs(path, exist_ok=True)
+        px = (field.copy() * self.ct_coef).astype("uint16")
+        for i in range(px.shape[2]):
+            write_slice(os.path.join(path, f"{i:04}.dcm"), px[:, :, i], self.voxelsize)
 
     def _calc_microarchitecture(self, peaks):
         # per slice the reference averages the peaks over its rows, then sums the slices

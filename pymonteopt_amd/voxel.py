@@ -445,8 +445,8 @@ class VoxelTuringModel(BaseVoxelMonteCarlo):
     def build(self, *initial_data, **kwargs):
         if initial_data or kwargs:
             self.model.set_params(*initial_data, **kwargs)
-        if self.bone_model is None:
-            raise ValueError("set_model(u) first")
+        if self.bone_model is None and not self.model.params.get("dicom_path"):
+            raise ValueError("set_model(u) first, or give dicom_path")
         self.model.build(self.bone_model)
         self.bone_model = None
         self._walker = None

@@ -57,8 +57,18 @@ def test_modeling_end_to_end(tmp_path):
     assert abs(U.mean() - t.bv_tv_real) < 1e-12
     info = json.load(open(tmp_path / "_info.json"))
     assert info["model_shape"] == [n, n, n] and info["microarchitecture"]["BV/TV real"] == pytest.approx(t.bv_tv_real)
-    with pytest.raises(NotImplementedError):
-        t.modeling(str(tmp_path), save_dicom=True)
+    # save_dicom: one 16-bit slice per z level of the field times ct_coef
+    from pymonteopt_amd import dicom
+    t2 = model(reps=5)
+    u, _ = t2.grow()
+    field = u.cpu().numpy()
+    t2.modeling(str(tmp_path / "dcm"), save_dicom=True)
+    files = sorted(p.name for p in (tmp_path / "dcm").glob("*.dcm"))
+    assert len(files) == n and files[0] == "0000.dcm"
+    sl = dicom.read_file(str(tmp_path / "dcm" / files[3]))
+    want = (field * t2.ct_coef).astype("uint16")[:, :, 3]
+    assert np.array_equal(sl.pixel_array, np.frombuffer(want.tobytes(), np.uint16).reshape(n, n))
+    assert sl.PixelSpacing[0] == pytest.approx(t2.voxelsize)
 
 
 @pytest.mark.parametrize("name", ["TuringModel_Rectangular", "TuringModel_Cylinder", "TuringModel_RnC"])
