@@ -23,8 +23,8 @@ from .geometry import (Source, center_add, fresnel_oblique, optics_table, plate_
                        validate_volume)
 from .runtime import TallySpec, Walker
 
-__all__ = ["set_params", "PlateModel", "LabelVolumeModel", "BaseVoxelMonteCarlo", "VoxelPlateModel",
-           "VoxelLabelModel", "VoxelTuringModel", "normalise"]
+__all__ = ["set_params", "PlateModel", "PlateExModel", "LabelVolumeModel", "BaseVoxelMonteCarlo",
+           "VoxelPlateModel", "VoxelPlateExModel", "VoxelLabelModel", "VoxelTuringModel", "normalise"]
 
 
 def set_params(data, keys, *initial_data, **kwargs):
@@ -102,6 +102,70 @@ class PlateModel(VoxelModel):
     def optics(self):
         return optics_table(self.params["ma"], self.params["ms"], self.params["g"], self.params["n"],
                             self.params["n_air"])
+
+    @property
+    def n_ambient(self):
+        return float(self.params["n_air"])
+
+    @property
+    def n_first(self):
+        return float(self.params["n"][0])
+
+
+class PlateExModel(VoxelModel):
+    """Glass around an intralipid box (pymopt/voxel_gpu/_classes.py:470-551): label 1 fills
+    int(x_size/l) x int(y_size/l) x int(thickness[1]/l) voxels, label 0 (glass) wraps it by
+    int(thickness[0]/l) + 1 voxels on every side, and the outermost voxels are the end
+    label 2. With first_layer_clear the beam starts in the first intralipid voxel."""
+
+    def __init__(self):
+        self.model_name = "PlateExModel"
+        self.dtype_f = np.float32
+        self.glass, self.intra, self.end_point = 0, 1, 2
+        self.params = {
+            "x_size": 40, "y_size": 40, "voxel_space": 0.1, "thickness": [1.3, 40], "n": [1.5, 1.4], "n_air": 1.,
+            "ma": [1e-5, 0.02374], "ms": [1e-5, 0.02374], "g": [0.9, 0.9],
+        }
+        self.keys = list(self.params.keys())
+        self._param_instantiating()
+        self.voxel_model = np.zeros((3, 3, 3), dtype=np.uint8)
+
+    def _param_instantiating(self):
+        P, f = self.params, self.dtype_f
+        self.thickness = P["thickness"]
+        self.n = np.array(list(P["n"]) + [P["n_air"]]).astype(f)
+        self.ms = np.array(P["ms"]).astype(f)
+        self.ma = np.array(P["ma"]).astype(f)
+        self.g = np.array(P["g"]).astype(f)
+        self.voxel_space = P["voxel_space"]
+        self.x_size = int(P["x_size"] / P["voxel_space"])
+        self.y_size = int(P["y_size"] / P["voxel_space"])
+        self.z_size = int(P["thickness"][1] / P["voxel_space"])
+        self.num_pix = int(P["thickness"][0] / P["voxel_space"])
+
+    def build(self):
+        k = self.num_pix + 1
+        vm = np.full((self.x_size, self.y_size, self.z_size), self.intra, np.uint8)
+        vm = np.pad(vm, k, constant_values=self.glass)
+        vm[[0, -1], :, :] = self.end_point
+        vm[:, [0, -1], :] = self.end_point
+        vm[:, :, [0, -1]] = self.end_point
+        self.voxel_model = vm
+        self.getModelSize()
+
+    def set_params(self, *initial_data, **kwargs):
+        set_params(self.params, self.keys, *initial_data, **kwargs)
+        self._param_instantiating()
+
+    def get_params(self):
+        return {"th": self.thickness, "ms": self.ms, "ma": self.ma, "n": self.n, "g": self.g}
+
+    def get_second_layer_addz(self):
+        return self.num_pix + 1
+
+    def optics(self):
+        P = self.params
+        return optics_table(P["ma"], P["ms"], P["g"], P["n"], P["n_air"])
 
     @property
     def n_ambient(self):
@@ -226,10 +290,8 @@ class BaseVoxelMonteCarlo:
             raise ValueError("%s is not a permitted for factor. Please choose from %s." % (beam_type, beam_type_list))
         if threadnum > 1024:
             raise Exception("There are too many threads. threadnum < 1024.")
-        if initial_refrect_by_angle:
-            raise NotImplementedError("initial_refrect_by_angle is not supported by the device walk")
-        if first_layer_clear:
-            raise NotImplementedError("first_layer_clear needs a PlateExModel geometry, not provided")
+        if first_layer_clear and not hasattr(model, "get_second_layer_addz"):
+            raise ValueError("first_layer_clear needs a model with a clear first layer (PlateExModel)")
         self.beam_type = beam_type
         self.dtype = dtype
         self.dtype_f = dtype_f
@@ -275,18 +337,35 @@ class BaseVoxelMonteCarlo:
     def get_model_params(self):
         return self.model.get_params()
 
+    def _oblique(self):
+        return self.beam_angle != 0 and self.w_beam == 0
+
     def source(self) -> Source:
-        """Launch description, pymopt/voxel/_classes.py:107-213."""
+        """Launch description, pymopt/voxel_gpu/_classes.py:144-231 (_set_inital_add,
+        _set_inital_vector, _set_inital_w)."""
         vm = self.model.voxel_model
         ix, iy = center_add(vm.shape[0]), center_add(vm.shape[1])
-        iz = int(getattr(self.model, "launch_z", getattr(self.model, "params", {}).get("launch_z", 1)))
+        if self.first_layer_clear:
+            iz = int(self.model.get_second_layer_addz())
+        else:
+            iz = int(getattr(self.model, "launch_z", getattr(self.model, "params", {}).get("launch_z", 1)))
         n1, n2 = self.model.n_ambient, self.model.n_first
         direction = (0.0, 0.0, 1.0)
         rsp = specular(n1, n2)
-        if self.beam_angle != 0 and self.w_beam == 0:
+        if self._oblique():
             at, r = fresnel_oblique(float(self.beam_angle), n1, n2)
             direction = (float(np.sin(at)), 0.0, float(np.cos(at)))
             rsp = r if n1 != n2 else 0.0
+            if self.initial_refrect_by_angle:
+                rsp = 0.0              # the reflected share leaves as packets instead (start())
+        elif self.first_layer_clear:
+            # air | clear first layer (n[0]) | second layer (n[1]), with the multiple
+            # reflections inside the clear layer (_set_inital_w, :205-218)
+            n2, n3 = float(self.model.n[0]), float(self.model.n[1])
+            rsp = 0.0
+            if n1 != n2:
+                r1, r2 = specular(n1, n2), specular(n2, n3)
+                rsp = r1 + r2 * (1 - r1) ** 2 / (1 - r1 * r2)
         kind, radius = _abi.PMO_BEAM_PENCIL, 0.0
         if self.beam_type == "flat" and self.w_beam != 0:
             kind, radius = _abi.PMO_BEAM_FLAT, float(self.w_beam)
@@ -308,7 +387,15 @@ class BaseVoxelMonteCarlo:
         self.nPh = int(self.nPh)
         self.rand_seed = int(rand_seed)
         wk = self.walker()
-        tal = wk.tallies(self.spec, self.nPh)
+        # initial_refrect_by_angle (_set_inital_vector, :164-182): a share R(angle) of the
+        # packets is reflected at entry with w = 1 and v = (sin a, 0, -cos a); they join the
+        # results without walking
+        n_refl = 0
+        if self.initial_refrect_by_angle and self._oblique():
+            _, r = fresnel_oblique(float(self.beam_angle), self.model.n_ambient, self.model.n_first)
+            n_refl = int(np.random.default_rng([self.rand_seed, 0x5EF1]).binomial(self.nPh, r))
+        n_walk = self.nPh - n_refl
+        tal = wk.tallies(self.spec, n_walk)
         print("")
         print("###### Start (Random seed: %s) ######" % rand_seed)
         print("")
@@ -317,12 +404,15 @@ class BaseVoxelMonteCarlo:
         ev1 = torch.cuda.Event(enable_timing=True)
         t0 = time.time()
         ev0.record()
-        wk.walk(self.source(), self.nPh, self.rand_seed, tal, fp64=self.fp64,
-                normal_fresnel=self.normal_fresnel, block_size=bs)
+        if n_walk > 0:
+            wk.walk(self.source(), n_walk, self.rand_seed, tal, fp64=self.fp64,
+                    normal_fresnel=self.normal_fresnel, block_size=bs)
         ev1.record()
         ev1.synchronize()
         self.time_s = ev0.elapsed_time(ev1) * 1e-3
         raw = tal.to_host(shape=wk.shape)
+        if n_refl:
+            self._add_entry_reflections(raw, n_refl)
         self._raw = raw
         self._result = normalise(raw, self.nPh, self.spec, wk.pitch, self.model.optics(), shape=wk.shape)
         if "absorbed" in self._result:
@@ -337,6 +427,24 @@ class BaseVoxelMonteCarlo:
         self.getRdTtRate()
         print("Calculation time: %0.3f seconds (walk %0.3f s on the GPU)." % (time.time() - t0, self.time_s))
         return self
+
+    def _add_entry_reflections(self, raw, n):
+        """n packets of weight 1 leaving the top face centre at (sin a, 0, -cos a)."""
+        a = float(self.beam_angle)
+        s = raw["scalars"]
+        for k in ("Rd", "face_z-", "w0"):
+            s[k] += float(n)
+        if "events" in raw:
+            raw["events"]["packets"] += n
+            raw["events"]["escaped"] += n
+        if "rd_a" in raw and self.spec.na > 0:
+            b = int(np.ceil(a / (np.pi / (2 * self.spec.na)))) - 1
+            if 0 <= b < self.spec.na:
+                raw["rd_a"][b] += float(n)
+        if "exit_state" in raw:
+            extra = np.zeros((7, n), np.float32)
+            extra[3], extra[5], extra[6] = np.sin(a), -np.cos(a), 1.0
+            raw["exit_state"] = np.concatenate([extra, raw["exit_state"]], axis=1)
 
     def _fluence(self, absorbed_raw_interior):
         """Phi[v] = A[v] / (mu_a[label v] * l^3 * nPh) in mm^-2 (zero where mu_a = 0)."""
@@ -451,3 +559,18 @@ class VoxelTuringModel(BaseVoxelMonteCarlo):
         self.bone_model = None
         self._walker = None
         return self
+
+
+class VoxelPlateExModel(BaseVoxelMonteCarlo):
+    """pymopt/voxel_gpu/_classes.py:843-886: the glass-wrapped intralipid box, the beam
+    starting under the clear glass by default."""
+
+    def __init__(self, *, nPh=1000, dtype_f=np.float32, dtype=np.int32, beam_type="TEM00", w_beam=0, beam_angle=0,
+                 initial_refrect_by_angle=False, first_layer_clear=True, threadnum=128, **tally_kwargs):
+        super().__init__(nPh=nPh, model=PlateExModel(), dtype_f=dtype_f, dtype=dtype, w_beam=w_beam,
+                         beam_angle=beam_angle, beam_type=beam_type,
+                         initial_refrect_by_angle=initial_refrect_by_angle, first_layer_clear=first_layer_clear,
+                         threadnum=threadnum, **tally_kwargs)
+
+    def set_params(self, *initial_data, **kwargs):
+        self.model.set_params(*initial_data, **kwargs)
