@@ -206,6 +206,55 @@ def turing_leg():
             "cell_steps_per_s": n ** 3 * steps / t.time_s, "bv_tv_real": bv, "peaks": peaks}
 
 
+def virtual_obd_leg(n_packets=10_000_000, seed=4):
+    """One subject of the virtual OBD pipeline at the script's sizes (pymopt/virtualOBD.py):
+    a 300^3 Turing volume grown 100 steps and tiled 2 x 4 x 2, wrapped in the subject's
+    cortical, subcutis and dermis layers, 1e7 packets walked with the three detector
+    tallies, the run and the three detector CSVs written. Times per stage, in seconds."""
+    import shutil
+    import tempfile
+
+    import numpy as np
+    import torch
+    from pymonteopt_amd import virtual_obd as vo
+    from pymonteopt_amd.param_generator import generate_variable_params
+    from pymonteopt_amd.voxel import VoxelTuringModel
+    root = tempfile.mkdtemp(prefix="pmo_vobd_")
+    try:
+        mp, mo = dict(vo.model_params), dict(vo.monte_params)
+        np.random.seed(seed)
+        t0 = time.perf_counter()
+        g = generate_variable_params()
+        g.set_params(vo.determining_params_range())
+        vp = g.generate(1)
+        u, bv = vo.generate_bone_model(vp["bv_tv"][0], os.path.join(root, "model"), mp)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        vp["bv_tv"][0] = bv
+        vo.subject_params(vp, mo)
+        m = VoxelTuringModel(nPh=n_packets, model_name="TuringModel", nr=vo.RD_BINS, dr=vo.RD_WIDTH,
+                             nl=vo.LINE_BINS, dl=vo.LINE_WIDTH, line_y=vo.LINE_Y, fluence=False)
+        m.set_model(u)
+        m.build(**{k: v for k, v in mo.items() if k not in ("symmetrization", "enclosure")})
+        shape = m.model.voxel_model.shape
+        t2 = time.perf_counter()
+        m.walker()
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        m.start(0)
+        t4 = time.perf_counter()
+        m.save_result(os.path.join(root, "monte"), coment="bench")
+        vo.calc_ray_tracing(m.get_result(), mo, os.path.join(root, "opt"), "bench")
+        t5 = time.perf_counter()
+        res = m.get_result()
+        return {"voxels": list(shape), "packets": n_packets, "bv_tv_real": bv,
+                "bone_model_s": t1 - t0, "wrap_s": t2 - t1, "upload_s": t3 - t2, "walk_s": m.time_s,
+                "start_s": t4 - t3, "save_and_csv_s": t5 - t4, "subject_s": t5 - t0,
+                "walk_photons_per_s": n_packets / m.time_s, "Rd": res["Rd"], "Tt": res["Tt"]}
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+
+
 def cpu_baseline_turing(n=96, steps=3):
     """The oracle's numpy restatement (float64 FFTs, one process) on an n^3 grid."""
     from oracle import turing as tp
@@ -412,6 +461,7 @@ def main():
         out["adjoint"] = adjoint_leg()
         out["obd"], obd_sample = obd_leg()
         out["turing"] = turing_leg()
+        out["virtual_obd"] = virtual_obd_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl)
         if not args.no_extra:

@@ -1,0 +1,82 @@
+"""GPU: the virtual OBD pipeline (pymopt/virtualOBD.py) end to end on a reduced subject,
+and the walk's detector tallies against the reference's host binning of the exit dump of
+the same run."""
+import bz2
+import json
+import os
+import pickle
+
+import numpy as np
+import pandas as pd
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# a 150^3 Turing volume (grid 25 x length 6), untiled, cropped to 3 mm: about 500 x 152 x 500
+# voxels once wrapped, against 958 x 1202 x 958 at the script's sizes
+SMALL_MODEL = {"grid": 25, "dx": 1 / 25, "length": 6, "tile_num_xz": 1, "tile_num_y": 1}
+SMALL_XZ = 3.0
+
+
+def _params():
+    from pymonteopt_amd import virtual_obd as vo
+    return dict(vo.model_params, **SMALL_MODEL), dict(vo.monte_params, xz_size=SMALL_XZ)
+
+
+def _subject(seed):
+    from pymonteopt_amd import virtual_obd as vo
+    from pymonteopt_amd.param_generator import generate_variable_params
+    np.random.seed(seed)
+    g = generate_variable_params()
+    g.set_params(vo.determining_params_range())
+    return g.generate(1)
+
+
+def test_calc_end_to_end(tmp_path):
+    from pymonteopt_amd import virtual_obd as vo
+    mp, mo = _params()
+    np.random.seed(4)            # rand() = 0.967: the normal population, a valid draw
+    alias = vo.calc(3, root=str(tmp_path), nPh=200000, model_params=mp, monte_params=mo)
+    assert alias is not None and alias.endswith("_it0003")
+    info = json.load(open(tmp_path / "model_result" / (alias + "_dicom") / "_info.json"))
+    assert 0.02 < info["microarchitecture"]["BV/TV real"] < 0.5
+    assert mo["bv_tv"] == pytest.approx(info["microarchitecture"]["BV/TV real"])   # the realised BV/TV is walked
+    with bz2.open(tmp_path / "monte_result" / (alias + "_LID.pkl.bz2"), "rb") as fp:
+        res = pickle.loads(fp.read())
+    assert res["nPh"] == 200000 and 0 < res["Rd"] < 1 and 0 <= res["Tt"] < 1
+    e = res["energy"]
+    assert abs(e["w0"] - (e["Rd"] + e["Tt"] + e["other"] + e["A"] + e["kill"] - e["gain"])) < 1e-9
+    run_info = json.load(open(tmp_path / "monte_result" / (alias + "_info.json")))
+    assert run_info["rand_seed"] == 3 and run_info["model"]["model_name"] == "TuringModel_Rectangular"
+    curves = {}
+    for suffix, n in (("_B", 300), ("_F", 300), ("_L", 400)):
+        df = pd.read_csv(tmp_path / "opt_result" / (alias + suffix + ".csv"), index_col=0)
+        assert list(df.columns) == [alias + suffix] and len(df) == n
+        assert np.all(df.values >= 0)
+        curves[suffix] = df.iloc[:, 0].values
+    # the top-face curve integrates to at most the diffuse reflectance; it peaks near the beam
+    i = np.arange(300)
+    rd_top = float((curves["_B"] * 2 * np.pi * (i + 0.5) * 0.1 ** 2).sum())
+    assert 0 < rd_top <= res["Rd"] + 1e-12
+    assert int(np.argmax(curves["_B"])) < 20
+    assert curves["_L"].sum() > 0
+
+
+def test_device_detectors_match_the_exit_dump(tmp_path):
+    from pymonteopt_amd import virtual_obd as vo
+    from pymonteopt_amd.voxel import VoxelTuringModel
+    mp, mo = _params()
+    vp = _subject(4)
+    u, _ = vo.generate_bone_model(vp["bv_tv"][0], str(tmp_path / "model"), mp)
+    vo.subject_params(vp, mo)
+    model = VoxelTuringModel(nPh=100000, model_name="TuringModel", nr=300, dr=0.1, nl=400, dl=0.1, line_y=5,
+                             fluence=False, exit_state=True)
+    model.set_model(u)
+    model.build(**{k: v for k, v in mo.items() if k not in ("symmetrization", "enclosure")})
+    res = model.start(11).get_result()
+    dev = vo.detector_curves({k: res[k] for k in ("r", "z", "Rd_top_r", "Tt_r", "line_z", "nPh")})
+    host = vo.detector_curves({"p": res["p"], "v": res["v"], "w": res["w"], "nPh": res["nPh"]})
+    for (ad, yd), (ah, yh), name in zip(dev, host, "BFL"):
+        assert np.array_equal(ad, ah), name
+        assert np.allclose(yd, yh, rtol=1e-5, atol=1e-12), (name, np.abs(yd - yh).max())
+        assert yd.sum() > 0, name

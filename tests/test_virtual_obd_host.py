@@ -1,0 +1,134 @@
+"""Host side of the virtual OBD pipeline (pymopt/virtualOBD.py, pymopt/utils/param_generator.py):
+population choice, subject draws, the mapping onto the bone-model keys and the detector
+CSVs, on hand-made inputs."""
+import os
+
+import numpy as np
+import pandas as pd
+import pytest
+
+from pymonteopt_amd import virtual_obd as vo
+from pymonteopt_amd.param_generator import generate_variable_params
+
+
+@pytest.mark.parametrize("seed,which", [(5, "osteoporosis"), (0, "osteopenia"), (4, "norm")])
+def test_population_thresholds(seed, which):
+    # np.random.seed(s); rand() -> 0.222 (s=5), 0.549 (s=0), 0.967 (s=4): a <= 0.4 is
+    # osteoporosis, a >= 0.6 normal, between osteopenia (virtualOBD.py:47-60)
+    np.random.seed(seed)
+    got = vo.determining_params_range()
+    assert got is getattr(vo, "range_params_" + which)
+
+
+def test_populations_differ_only_in_bone():
+    for r in (vo.range_params_osteoporosis, vo.range_params_osteopenia):
+        assert {k for k in r if r[k] != vo.range_params_norm[k]} == {"bv_tv", "th_cortical"}
+    assert vo.range_params_osteoporosis["bv_tv"] == [0.085, 0.022]
+    assert vo.range_params_osteopenia["th_cortical"] == [0.571, 0.173]
+
+
+def test_uniform_draws_in_order_and_range():
+    g = generate_variable_params()
+    np.random.seed(11)
+    vp = g.generate(4)
+    np.random.seed(11)
+    u = np.random.rand(6, 4)            # dermis th, ma, msd, then subcutis th, ma, msd
+    order = ["th_dermis", "ma_dermis", "msd_dermis", "th_subcutaneus", "ma_subcutaneus", "msd_subcutaneus"]
+    for row, k in zip(u, order):
+        lo, hi = g.params[k]
+        assert np.allclose(vp[k], lo + row * (hi - lo), rtol=0, atol=1e-15)
+        assert np.all((vp[k] >= lo) & (vp[k] <= hi))
+    assert g.get_variable_params() is vp
+
+
+def test_bone_pair_is_correlated_normal_with_std():
+    # the second entry of bv_tv / th_cortical is the standard deviation, corr the correlation
+    g = generate_variable_params()
+    g.set_params(bv_tv=[0.2, 0.03], th_cortical=[0.8, 0.15], corr=0.54)
+    np.random.seed(3)
+    vp = g.generate(200000)
+    b, c = vp["bv_tv"], vp["th_cortical"]
+    assert abs(b.mean() - 0.2) < 4 * 0.03 / np.sqrt(2e5)
+    assert abs(c.mean() - 0.8) < 4 * 0.15 / np.sqrt(2e5)
+    assert b.std() == pytest.approx(0.03, rel=0.01)
+    assert c.std() == pytest.approx(0.15, rel=0.01)
+    assert np.corrcoef(b, c)[0, 1] == pytest.approx(0.54, abs=0.01)
+
+
+def test_unknown_key_raises():
+    with pytest.raises(KeyError):
+        generate_variable_params().set_params(ma_marrow=[0.005, 0.012])
+
+
+def test_subject_params_mapping():
+    vp = {"th_dermis": [1.5], "ma_dermis": [0.02], "msd_dermis": [2.0], "th_subcutaneus": [3.0],
+          "ma_subcutaneus": [0.01], "msd_subcutaneus": [1.2], "bv_tv": [0.12], "th_cortical": [0.7]}
+    p = vo.subject_params(vp, dict(vo.monte_params))
+    assert p["ms_dermis"] == pytest.approx(20.0)          # mu_s' / (1 - g), g = 0.9
+    assert p["ms_subcutaneus"] == pytest.approx(12.0)
+    assert p["ms_space"] == pytest.approx(12.0) and p["ma_space"] == 0.01   # marrow = subcutis optics
+    assert (p["th_dermis"], p["th_subcutaneus"], p["bv_tv"], p["th_cortical"]) == (1.5, 3.0, 0.12, 0.7)
+
+
+def test_monte_params_fit_the_rectangular_model_without_unverified_keys():
+    from pymonteopt_amd.bone_models import TuringModel_Rectangular
+    vp = {k: [1.0] for k in ("th_dermis", "ma_dermis", "msd_dermis", "th_subcutaneus", "ma_subcutaneus",
+                             "msd_subcutaneus", "bv_tv", "th_cortical")}
+    p = vo.subject_params(vp, dict(vo.monte_params))
+    m = TuringModel_Rectangular()
+    with pytest.raises(KeyError):
+        m.set_params(**p)
+    m.set_params(**{k: v for k, v in p.items() if k not in ("symmetrization", "enclosure")})
+    assert m.params["voxel_space"] == vo.model_params["voxelsize"]
+    assert m.params["n_trabecular"] == 1.55
+
+
+def _dump():
+    # exits: (x, y, z, vx, vy, vz, w)
+    rows = [
+        (0.03, 0.04, 0.0, 0.0, 0.0, -1.0, 0.5),     # top face, r = 0.05 -> B bin 0
+        (0.25, 0.0, 0.0, 0.1, 0.0, -0.9, 0.25),     # top face, r = 0.25 -> B bin 2
+        (0.0, 0.3, 0.0, 0.0, 0.2, 0.9, 0.125),      # v_z > 0 at the top face: F bin 2 (parse quirk)
+        (1.0, 0.0, 9.0, 0.0, 0.0, 1.0, 0.0625),     # far face -> F bin 9
+        (-8.0, 1.0, 1.25, -1.0, 0.0, 0.0, 1.0),     # side exit v_x < 0, z = 1.25 -> L bin 12
+        (-8.0, 6.0, 2.05, -1.0, 0.0, 0.0, 1.0),     # |y| >= 5: not on the line detector
+        (0.0, 0.1, 0.5, 0.0, 0.0, -1.0, 2.0),       # v_z < 0 below the top face: not in B
+    ]
+    a = np.array(rows, np.float64).T
+    return {"p": a[0:3], "v": a[3:6], "w": a[6], "nPh": 10}
+
+
+def test_detector_curves_from_dump_by_hand():
+    (rb, B), (rf, F), (rl, L) = vo.detector_curves(_dump())
+    assert B.size == F.size == 300 and L.size == 400
+    da = lambda i: 2 * np.pi * (i + 0.5) * 0.1 ** 2 * 10
+    assert B[0] == pytest.approx(0.5 / da(0)) and B[2] == pytest.approx(0.25 / da(2))
+    assert np.count_nonzero(B) == 2
+    assert F[2] == pytest.approx(0.125 / da(2)) and F[9] == pytest.approx(0.0625 / da(9))
+    assert np.count_nonzero(F) == 2
+    assert L[12] == pytest.approx(1.0 / (0.1 * 5 * 2 * 10)) and np.count_nonzero(L) == 1
+    assert rb[3] == pytest.approx(0.3) and rl[-1] == pytest.approx(39.9)
+
+
+def test_detector_curves_prefer_device_tallies():
+    nr, nl = 300, 400
+    res = {"r": np.arange(nr) * 0.1, "z": np.arange(nl) * 0.1, "Rd_top_r": np.full(nr, 1.0),
+           "Tt_r": np.full(nr, 2.0), "line_z": np.full(nl, 3.0), "nPh": 10}
+    (_, B), (_, F), (_, L) = vo.detector_curves(res)
+    assert np.all(B == 1.0) and np.all(F == 2.0) and np.all(L == 3.0)
+    res["r"] = np.arange(nr) * 0.2          # other binning and no dump: refused
+    with pytest.raises(ValueError):
+        vo.detector_curves(res)
+
+
+def test_calc_ray_tracing_csv_layout(tmp_path):
+    path = str(tmp_path / "opt" / "subject")
+    vo.calc_ray_tracing(_dump(), vo.monte_params, path, "subject")
+    (rb, B), (rf, F), (rl, L) = vo.detector_curves(_dump())
+    for suffix, alpha, y in (("_B", rb, B), ("_F", rf, F), ("_L", rl, L)):
+        fn = path + suffix + ".csv"
+        assert os.path.exists(fn)
+        df = pd.read_csv(fn, index_col=0)
+        assert list(df.columns) == ["subject" + suffix]
+        assert np.allclose(df.index.values, alpha, rtol=0, atol=1e-12)
+        assert np.allclose(df.iloc[:, 0].values, y, rtol=1e-12, atol=0)
