@@ -8,6 +8,7 @@ timed region runs K walks (seeds 0..K-1) back to back. Prints one JSON line.
 from __future__ import annotations
 
 import argparse
+import contextlib
 import json
 import math
 import os
@@ -456,18 +457,21 @@ def main():
         scale = N / float(prof["packets_per_launch"])
         out["roofline"]["traffic"] = prof["hbm_bytes_per_launch"] * scale
         out["roofline"]["traffic_source"] = prof["source"]
-    if not args.no_extra:
-        out["batched"] = batched_leg()
-        out["adjoint"] = adjoint_leg()
-        out["obd"], obd_sample = obd_leg()
-        out["turing"] = turing_leg()
-        out["virtual_obd"] = virtual_obd_leg()
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(wl)
+    # the legs' own progress prints (the model API prints as the reference's does) go to
+    # stderr, so stdout carries the one JSON line only
+    with contextlib.redirect_stdout(sys.stderr):
         if not args.no_extra:
-            out["obd"]["cpu_baseline"] = cpu_baseline_obd(*obd_sample)
-            out["obd"]["gpu_vs_cpu"] = out["obd"]["traces_per_s"] / out["obd"]["cpu_baseline"]["value"]
-            out["turing"]["cpu_baseline"] = cpu_baseline_turing()
+            out["batched"] = batched_leg()
+            out["adjoint"] = adjoint_leg()
+            out["obd"], obd_sample = obd_leg()
+            out["turing"] = turing_leg()
+            out["virtual_obd"] = virtual_obd_leg()
+        if not args.no_cpu:
+            out["cpu_baseline"] = cpu_baseline(wl)
+            if not args.no_extra:
+                out["obd"]["cpu_baseline"] = cpu_baseline_obd(*obd_sample)
+                out["obd"]["gpu_vs_cpu"] = out["obd"]["traces_per_s"] / out["obd"]["cpu_baseline"]["value"]
+                out["turing"]["cpu_baseline"] = cpu_baseline_turing()
     print(json.dumps(out), flush=True)
 
 

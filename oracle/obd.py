@@ -164,8 +164,10 @@ def refract(v, nn, ni, nt):
 
 
 def plane_hit(p, v, z):
-    a = (z - p[2]) / v[2]
-    return p + a * v
+    # rays parallel to the plane give inf/nan here; the callers drop them by their masks
+    with np.errstate(divide="ignore", invalid="ignore"):
+        a = (z - p[2]) / v[2]
+        return p + a * v
 
 
 def sphere_hit(p, v, center, r, outward):
