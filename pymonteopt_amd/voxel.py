@@ -399,7 +399,9 @@ class BaseVoxelMonteCarlo:
         print("")
         print("###### Start (Random seed: %s) ######" % rand_seed)
         print("")
-        bs = self.threadnum if self.threadnum % 64 == 0 else 0
+        # threadnum sized the reference's CUDA blocks (one packet per thread); the persistent
+        # walk picks its own block size and LDS window from the run (block_size 0)
+        bs = 0
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
         t0 = time.time()

@@ -224,6 +224,30 @@ def test_tally_window_placement_is_invisible(slab):
         gpu_run(wk, wl, N, 11, window=(64, 64, 32))
 
 
+def test_wide_launch_is_invisible(slab):
+    """Runs of 2^20 packets or more with an absorbed map default to 1024-lane blocks and a
+    28x28x24 LDS window; a default window shrinks to fit beside a large r-z histogram.
+    Neither changes any packet's path: tallies agree with a 256-lane run without a
+    window to FP64 summation order."""
+    from pymonteopt_amd.runtime import TallySpec
+    wl, wk = slab
+    N = (1 << 20) + 4321
+    base = gpu_run(wk, wl, N, 17, block_size=256, window=(-1, -1, -1))
+    for kw in ({}, {"block_size": 1024, "window": (20, 20, 10)}, {"block_size": 512}):
+        r = gpu_run(wk, wl, N, 17, **kw)
+        path_events = lambda e: {k: v for k, v in e.items() if k != "global_deposits"}
+        assert path_events(r["events"]) == path_events(base["events"]), kw
+        assert r["events"]["global_deposits"] < base["events"]["global_deposits"], kw
+        assert np.allclose(r["absorbed"], base["absorbed"], rtol=1e-10, atol=1e-15), kw
+        for k in ("Rd", "Tt", "A"):
+            assert r["scalars"][k] == pytest.approx(base["scalars"][k], rel=1e-12), kw
+    # 64 x 128 r-z bins take 64 KiB of LDS: the default wide window shrinks beside them
+    spec = TallySpec(**dict(wl.tallies, rz=(64, 0.1, 128, 0.05)))
+    r = gpu_run(wk, wl, N, 17, spec=spec)
+    assert np.allclose(r["absorbed"], base["absorbed"], rtol=1e-10, atol=1e-15)
+    assert r["fluence_rz"].sum() > 0
+
+
 def test_event_scheduling_is_invisible(slab):
     """Which step a wave runs first (quorum) changes only the interleaving of lanes, not
     any packet's own event sequence: tallies agree to FP64 summation order."""
