@@ -117,12 +117,16 @@ typedef struct pmo_run {
     float wth;              /* roulette threshold (reference 1e-4)                             */
     float roulette_m;       /* roulette survival 1/m, weight multiplier m (reference 10)       */
     uint32_t flags;         /* PMO_F_*                                                         */
-    int32_t block_size;     /* threads per block, multiple of 64, <= 1024; 0 = default         */
+    int32_t block_size;     /* threads per block, multiple of 64, <= 1024; 0 = default (1024 for
+                               runs of >= 2^20 packets with an absorbed map, else 256)          */
     int32_t grid_blocks;    /* persistent blocks; 0 = fill the device at full occupancy        */
     int32_t packets_per_lane; /* a wave claims 64*this packet ids per work-counter atomic; 0 = default */
     int32_t window[3];      /* extents (x, y, z) of the box of the absorbed tally that each block
                                accumulates in LDS, centred laterally on the launch voxel and
-                               starting at its z; {0,0,0} = default, {-1,-1,-1} = none          */
+                               starting at its z; {-1,-1,-1} = none; {0,0,0} = default: 28x28x24
+                               with 1024-lane blocks (the default block size for runs of >= 2^20
+                               packets with an absorbed map), else 16x16x8, shrunk to the LDS the
+                               histograms leave                                                 */
     const uint32_t* key_dev; /* device (seed, stream) read by the kernel instead of seed/stream when
                                 non-NULL, so a captured HIP graph can be replayed with new keys */
     int32_t quorum;          /* a wave runs its interaction step once this many of its 64 lanes are
@@ -156,7 +160,8 @@ typedef struct pmo_tallies {
     double* fluence_rz;     /* device [rz_nr][rz_nz]: sum of deposit / mu_a at each interaction,
                                the r-z fluence pymopt/fluence/intarnal_fluence.py Fluence2D intends
                                (pymopt/voxel_unverified/_classes.py:394-399), or NULL. Held in LDS
-                               per block: rz_nr*rz_nz*8 B counts against the 64 KiB budget     */
+                               per block: rz_nr*rz_nz*8 B counts against the block's LDS (the
+                               device's per-block limit, 160 KiB on gfx950)                     */
 } pmo_tallies;
 
 /* ---- batched plates ----------------------------------------------------------------- */
