@@ -207,6 +207,40 @@ def turing_leg():
             "cell_steps_per_s": n ** 3 * steps / t.time_s, "bv_tv_real": bv, "peaks": peaks}
 
 
+def bone_leg(peaks, n_packets=100_000_000):
+    """The 256^3 cancellous-bone phantom at 0.02 mm (GRF trabecular core, cortical shell,
+    soft-tissue top): one launch of 1e8 packets with the absorbed map, Rd(r) and the ten
+    annular top-face detectors, after a 1e6-packet warm-up. Fresnel at every bone/marrow
+    face makes this the crossing- and Fresnel-heavy case."""
+    import torch
+    from pymonteopt_amd import _abi, presets, roofline
+    from pymonteopt_amd.runtime import TallySpec, Walker
+    wl = presets.bone_workload(seed=0)
+    wk = Walker(wl.labels, wl.pitch, wl.optics)
+    tal = wk.tallies(TallySpec(**wl.tallies))
+    wk.walk(wl.source, 1_000_000, 99, tal)
+    tal.zero()
+    torch.cuda.synchronize()
+    e0, e1 = _events()
+    e0.record()
+    wk.walk(wl.source, n_packets, 5, tal)
+    e1.record()
+    e1.synchronize()
+    t = e0.elapsed_time(e1) * 1e-3
+    s = dict(zip(_abi.SCALAR_NAMES, tal.scalars.cpu().tolist()))
+    ev = dict(zip(_abi.EVENT_NAMES, [int(x) for x in tal.events.cpu().tolist()]))
+    ops = roofline.ops_per_packet(ev)
+    glob_pp = ev["global_deposits"] / n_packets
+    rings = tal.to_host()["ring_w"] / n_packets
+    return {"workload": wl.name + f", {n_packets:.0e} packets", "seconds": t, "photons_per_s": n_packets / t,
+            "events_per_packet": {k: ev[k] / n_packets for k in ("interactions", "crossings", "fresnel", "roulette")},
+            "Rd": s["Rd"] / n_packets, "Tt": s["Tt"] / n_packets, "A": s["A"] / n_packets,
+            "ring_weight_per_packet": [float(x) for x in rings],
+            "valu_frac_published": n_packets * ops / t / roofline.PEAK_VALU_PUBLISHED_LANE_OPS,
+            "atomic_frac_measured": glob_pp * n_packets / t / peaks["fp64_atomics_per_s"],
+            "global_deposits_per_packet": glob_pp, "ops_per_packet": ops}
+
+
 def virtual_obd_leg(n_packets=10_000_000, seed=4):
     """One subject of the virtual OBD pipeline at the script's sizes (pymopt/virtualOBD.py):
     a 300^3 Turing volume grown 100 steps and tiled 2 x 4 x 2, wrapped in the subject's
@@ -466,6 +500,7 @@ def main():
             out["obd"], obd_sample = obd_leg()
             out["turing"] = turing_leg()
             out["virtual_obd"] = virtual_obd_leg()
+            out["bone"] = bone_leg(peaks)
         if not args.no_cpu:
             out["cpu_baseline"] = cpu_baseline(wl)
             if not args.no_extra:
