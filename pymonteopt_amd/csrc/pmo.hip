@@ -399,7 +399,10 @@ int diff_launch(const pmo_volume* vol, const pmo_source* src, const pmo_run* run
     if (mode == 1 && (!diff->exit_rec || !diff->coef || !diff->grad_mua))
         return fail(PMO_ERR_INVALID, "adjoint needs exit_rec, coef and grad_mua");
     if (run->n_packets == 0) return PMO_OK;
-    const int bs = run->block_size > 0 ? run->block_size : kDefaultBlock;
+    // the replay deposits the per-voxel gradient as the walk deposits the absorbed map, so
+    // large replays take the same wide launch and LDS box
+    const bool wide = mode == 1 && run->n_packets >= kWideRunPackets;
+    const int bs = run->block_size > 0 ? run->block_size : (wide ? kWideBlock : kDefaultBlock);
     if (bs % 64 != 0 || bs > 1024) return fail(PMO_ERR_INVALID, "block_size must be a multiple of 64, <= 1024");
     DevInfo d;
     int rc = device_info(&d);
@@ -434,22 +437,6 @@ int diff_launch(const pmo_volume* vol, const pmo_source* src, const pmo_run* run
     P.mismatch = reinterpret_cast<unsigned long long*>(diff->mismatch);
     P.work = reinterpret_cast<unsigned long long*>(workspace);
     if (mode == 1 && !P.mismatch) P.mismatch = P.work + 2;   // scratch counter in the workspace
-    // replay: LDS box of the gradient map around the launch voxel (same as the absorbed tally)
-    if (mode == 1) {
-        int wdim[3] = {kDefaultWindow[0], kDefaultWindow[1], kDefaultWindow[2]};
-        const int ext[3] = {vol->nx, vol->ny, vol->nz};
-        const int ctr[3] = {src->ix, src->iy, src->iz};
-        int org[3];
-        for (int a = 0; a < 3; ++a) {
-            if (wdim[a] > ext[a]) wdim[a] = ext[a];
-            org[a] = a < 2 ? ctr[a] - wdim[a] / 2 : ctr[a];
-            if (org[a] < 0) org[a] = 0;
-            if (org[a] + wdim[a] > ext[a]) org[a] = ext[a] - wdim[a];
-        }
-        P.win_x0 = org[0]; P.win_y0 = org[1]; P.win_z0 = org[2];
-        P.win_nx = (uint32_t)wdim[0]; P.win_ny = (uint32_t)wdim[1]; P.win_nz = (uint32_t)wdim[2];
-    }
-    P.win_cells = P.win_nx * P.win_ny * P.win_nz;
     const uint32_t njac = P.jac ? P.n_labels : 0u;
     P.lds_tab = 0;
     P.lds_scal = (uint32_t)align_up((size_t)(vol->n_labels + 1) * sizeof(pmo::LabelConst<float>), 16);
@@ -457,8 +444,36 @@ int diff_launch(const pmo_volume* vol, const pmo_source* src, const pmo_run* run
     P.lds_hist = P.lds_ev + (uint32_t)align_up(pmo::kBE_COUNT * sizeof(unsigned long long), 16);
     P.lds_jac = P.lds_hist + (uint32_t)align_up((size_t)P.nr * sizeof(double), 16);
     P.lds_win = P.lds_jac + (uint32_t)align_up((size_t)P.nr * njac * 2u * sizeof(double), 16);
+    if (P.lds_win > d.max_lds) return fail(PMO_ERR_UNSUPPORTED, "differentiable walk needs %u B of LDS", P.lds_win);
+    // replay: LDS box of the gradient map around the launch voxel (as for the absorbed
+    // tally), shrunk depth first to the LDS left
+    if (mode == 1) {
+        const int* wdef = bs >= kWideBlock ? kWideWindow : kDefaultWindow;
+        int wdim[3] = {wdef[0], wdef[1], wdef[2]};
+        const int ext[3] = {vol->nx, vol->ny, vol->nz};
+        const int ctr[3] = {src->ix, src->iy, src->iz};
+        for (int a = 0; a < 3; ++a)
+            if (wdim[a] > ext[a]) wdim[a] = ext[a];
+        const size_t budget = (d.max_lds - P.lds_win) / sizeof(double);
+        while ((size_t)wdim[0] * wdim[1] * wdim[2] > budget) {
+            if (wdim[2] > wdim[0] / 2 && wdim[2] > 1) --wdim[2];
+            else if (wdim[0] > 1) { --wdim[0]; wdim[1] = wdim[1] > 1 ? wdim[1] - 1 : 1; }
+            else { wdim[2] = 0; break; }
+        }
+        if (wdim[2] > 0) {
+            int org[3];
+            for (int a = 0; a < 3; ++a) {
+                org[a] = a < 2 ? ctr[a] - wdim[a] / 2 : ctr[a];
+                if (org[a] < 0) org[a] = 0;
+                if (org[a] + wdim[a] > ext[a]) org[a] = ext[a] - wdim[a];
+            }
+            P.win_x0 = org[0]; P.win_y0 = org[1]; P.win_z0 = org[2];
+            P.win_nx = (uint32_t)wdim[0]; P.win_ny = (uint32_t)wdim[1]; P.win_nz = (uint32_t)wdim[2];
+        }
+    }
+    P.win_cells = P.win_nx * P.win_ny * P.win_nz;
     const size_t lds = P.lds_win + (size_t)P.win_cells * sizeof(double);
-    if (lds > kMaxLds) return fail(PMO_ERR_UNSUPPORTED, "differentiable walk needs %zu B of LDS", lds);
+    if (lds > d.max_lds) return fail(PMO_ERR_UNSUPPORTED, "differentiable walk needs %zu B of LDS", lds);
 
     int nb = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::diff_kernel<float>, bs, lds);
