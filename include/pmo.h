@@ -130,8 +130,8 @@ typedef struct pmo_run {
     const uint32_t* key_dev; /* device (seed, stream) read by the kernel instead of seed/stream when
                                 non-NULL, so a captured HIP graph can be replayed with new keys */
     int32_t quorum;          /* a wave runs its interaction step once this many of its 64 lanes are
-                                ready for one (else it advances face crossings); 0 = default:
-                                32 with 1024-lane blocks, else 48                               */
+                                ready for one (else it advances face crossings); 0 = default
+                                (32)                                                            */
 } pmo_run;
 
 typedef struct pmo_tallies {

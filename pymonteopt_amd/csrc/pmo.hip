@@ -49,10 +49,10 @@ constexpr int kDefaultWindow[3] = {16, 16, 8};
 constexpr int kWideBlock = 1024;
 constexpr int kWideWindow[3] = {28, 28, 24};
 constexpr uint64_t kWideRunPackets = 1ull << 20;
-// with 4 waves per SIMD a lower quorum keeps them busier: on the MI355X, 32 beats 48 by 11%
-// on the 256^3 bone phantom and ties on the skin plate and the slab
-constexpr int kWideQuorum = 32;
-constexpr int kDefaultQuorum = 48;
+// interactions run once 32 of a wave's 64 lanes want one. On the MI355X, against 48: +11%
+// on the 256^3 bone phantom with either launch shape, +2% on the skin plate without the
+// absorbed map, a tie with it, -2% on the slab
+constexpr int kDefaultQuorum = 32;
 
 struct DevInfo {
     int n_cu = 0;
@@ -187,7 +187,7 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     P.max_events = kMaxEvents;
     if (run->quorum < 0 || run->quorum > 64) return fail(PMO_ERR_INVALID, "quorum %d outside [0, 64]", run->quorum);
     // >= 1, so a wave whose lanes all wait for crossings still advances them
-    P.quorum = run->quorum > 0 ? (uint32_t)run->quorum : (uint32_t)(bs >= kWideBlock ? kWideQuorum : kDefaultQuorum);
+    P.quorum = run->quorum > 0 ? (uint32_t)run->quorum : (uint32_t)kDefaultQuorum;
     P.scalars = tal->scalars;
     P.events = reinterpret_cast<unsigned long long*>(tal->events);
     P.absorbed = tal->absorbed;
