@@ -362,8 +362,9 @@ def main():
     ap.add_argument("--quorum", type=int, default=0, help="lanes ready before a wave runs interactions")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
-              "(the walk shards by packet-id range; see the multi-GPU paragraph of DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported. The walk would shard "
+              "by packet-id range, one share per GPU, with one all-reduce of the FP64 tallies at the end.",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch

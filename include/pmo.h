@@ -249,7 +249,7 @@ int pmo_walk_forward_diff(const pmo_volume* vol, const pmo_source* src, const pm
 int pmo_walk_adjoint(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, const pmo_diff* diff,
                      void* workspace, void* stream);
 
-/* Diagnostics that re-measure the two peaks the walk is judged against (no reference
+/* Diagnostics that re-measure the two peaks the walk is measured against (no reference
  * counterpart). The caller times them on `stream` (e.g. with HIP events):
  * pmo_probe_atomics issues blocks*256*iters device-scope atomic adds (no return) of 1 to
  * pseudo-random addresses of the device array buf[n], whose element type `kind` names; with
