@@ -170,6 +170,35 @@ def test_skin_flat_beam_parity(torch_cuda, oracle):
         assert ledger_error(g["scalars"]) < 1e-6
 
 
+@pytest.mark.parametrize("variant", ["tem00", "oblique"])
+def test_beam_variant_parity(torch_cuda, oracle, variant):
+    """TEM00 beam of waist 0.5 mm on the skin plate, and a pencil refracted from 0.6 rad
+    incidence on the slab, 2e4 packets x 10 seeds against the oracle."""
+    import dataclasses
+    from pymonteopt_amd import _abi, presets
+    from pymonteopt_amd.geometry import fresnel_oblique
+    from pymonteopt_amd.runtime import Walker
+    if variant == "tem00":
+        wl = presets.skin_plate()
+        wl = dataclasses.replace(wl, source=dataclasses.replace(wl.source, kind=_abi.PMO_BEAM_GAUSS, radius=0.5))
+    else:
+        wl = presets.slab_plate()
+        at, r = fresnel_oblique(0.6, 1.0, 1.37)
+        src = dataclasses.replace(wl.source, dir=(float(np.sin(at)), 0.0, float(np.cos(at))),
+                                  w0=float(np.float32(1.0) - np.float32(r)))
+        wl = dataclasses.replace(wl, source=src)
+    wk = Walker(wl.labels, wl.pitch, wl.optics)
+    N = 20000
+    G = [gpu_run(wk, wl, N, 300 + s) for s in range(10)]
+    O = [oracle_run(oracle, wl, N, s, absorbed=False) for s in range(10)]
+    for k in ("Rd", "Tt", "A"):
+        assert zscore([g["scalars"][k] for g in G], [o["scalars"][k] for o in O]) < 3, k
+    zr = binned_z([g["rd_r"] for g in G], [o["rd_r"] for o in O])
+    assert np.sum(zr > 3) <= 2 and np.all(zr < 5)
+    for g in G:
+        assert ledger_error(g["scalars"]) < 1e-6
+
+
 def test_bone_phantom_parity(torch_cuda, oracle):
     """64^3 bone phantom (GRF trabecular core, Fresnel at every bone/marrow face)."""
     from pymonteopt_amd import geometry, presets
