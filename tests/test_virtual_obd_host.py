@@ -132,3 +132,22 @@ def test_calc_ray_tracing_csv_layout(tmp_path):
         assert list(df.columns) == ["subject" + suffix]
         assert np.allclose(df.index.values, alpha, rtol=0, atol=1e-12)
         assert np.allclose(df.iloc[:, 0].values, y, rtol=1e-12, atol=0)
+
+
+def test_utils_namespace(tmp_path, capsys):
+    import json
+    from pymonteopt_amd import utils
+    assert utils.generate_variable_params is generate_variable_params
+    utils.calTime(130.5, 10.0)
+    assert "2 minutes 0.500 seconds" in capsys.readouterr().out
+    utils.calTime(7500.0, 0.0)                 # 125 min -> 2 h 5 minutes
+    assert "2 h 5.000 minutes" in capsys.readouterr().out
+    s = json.dumps({"a": np.int32(3), "b": np.float32(0.5), "c": np.arange(3)}, cls=utils.ToJsonEncoder)
+    assert json.loads(s) == {"a": 3, "b": 0.5, "c": [0, 1, 2]}
+    with pytest.raises(TypeError):
+        json.dumps({"x": object()}, cls=utils.ToJsonEncoder)
+    utils.check_folder(str(tmp_path / "a" / "b"))
+    assert (tmp_path / "a" / "b").is_dir()
+    d = {"k": 1}
+    with pytest.raises(KeyError):
+        utils.set_params(d, ["k"], z=2)
