@@ -10,7 +10,7 @@ runs, only computed together.
 from __future__ import annotations
 
 import ctypes as C
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 import torch

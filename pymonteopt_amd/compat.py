@@ -15,7 +15,6 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import _abi
 from .geometry import Source
 from .runtime import TallySpec, Walker
 

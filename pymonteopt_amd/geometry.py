@@ -7,7 +7,7 @@ HIP walk reads: labels as bytes through L2, the table copied to LDS.
 """
 from __future__ import annotations
 
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 

@@ -272,7 +272,6 @@ def normalise(raw: dict, n_packets: int, spec: TallySpec, pitch: float, optics: 
         if interior:
             a = a[1:-1, 1:-1, 1:-1]
         out["absorbed"] = a / N
-        labels = None
         out["_absorbed_raw_interior"] = a
     return out
 

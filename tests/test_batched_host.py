@@ -2,7 +2,6 @@
 import ctypes as C
 
 import numpy as np
-import pytest
 
 from pymonteopt_amd import _abi
 from pymonteopt_amd.batched import Plate, slab_sweep

@@ -34,7 +34,6 @@ def test_fields_match_the_oracle(steps):
 
 
 def test_threshold_peaks_and_labels_match_the_oracle():
-    import torch
     t = model(reps=30)
     u, _ = t.grow()
     field = u.cpu().numpy().astype(np.float64)

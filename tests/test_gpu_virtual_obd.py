@@ -3,7 +3,6 @@ and the walk's detector tallies against the reference's host binning of the exit
 the same run."""
 import bz2
 import json
-import os
 import pickle
 
 import numpy as np

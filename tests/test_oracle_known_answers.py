@@ -5,7 +5,6 @@ import math
 import os
 
 import numpy as np
-import pytest
 
 from pymonteopt_amd.geometry import optics_table, plate_volume, specular
 
