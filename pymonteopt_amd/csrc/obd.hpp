@@ -92,19 +92,33 @@ __device__ __forceinline__ void obd_to_plane(float& x, float& y, float& z, float
     z = zp;
 }
 
-// Weight reaching the photodiode at one focus step (0 if the ray is lost).
+// Entry point on lens 1's plano face z = z_lens1 from the sample plane z = 0, with
+// ivz = 1 / vz precomputed per packet (the scan moves this plane only).
+__device__ __forceinline__ void obd_lens1_entry(const ObdStep& S, float& x, float& y, float vx, float vy, float ivz)
+{
+    const float a = S.z_lens1 * ivz;
+    x += a * vx;
+    y += a * vy;
+}
+
+// Weight reaching the photodiode at focus step s (0 if the ray is lost) of exit packet j,
+// whose direction (vx, vy, vz) is a unit vector. The Fresnel uniforms are drawn only for
+// rays that pass lens 1's aperture, where most rays end; the draws depend on (s, j) alone,
+// so the order changes nothing.
 __device__ float obd_trace(const ObdParams& P, const ObdStep& S, float x, float y, float vx, float vy, float vz,
-                           float w, const float u[4])
+                           float ivz, float w, int32_t s, unsigned long long j)
 {
     const bool ex = P.expect != 0;
-    float z = 0.f, R = 0.f, wt = w;
-    {
-        const float inv = __builtin_amdgcn_rsqf(vx * vx + vy * vy + vz * vz);
-        vx *= inv; vy *= inv; vz *= inv;
-    }
+    float z = S.z_lens1, R = 0.f, wt = w;
     // lens 1, plano face (air -> glass)
-    obd_to_plane(x, y, z, vx, vy, vz, S.z_lens1);
+    obd_lens1_entry(S, x, y, vx, vy, ivz);
     if (!(x * x + y * y <= P.outer1_sq)) return 0.f;
+    float u[4] = {0.f, 0.f, 0.f, 0.f};
+    if (!ex) {
+        const uint4 r = philox4x32_10(make_uint4((uint32_t)s, kTagObd, (uint32_t)j, (uint32_t)(j >> 32)), P.seed,
+                                      P.stream);
+        u[0] = u01<float>(r.x); u[1] = u01<float>(r.y); u[2] = u01<float>(r.z); u[3] = u01<float>(r.w);
+    }
     if (!obd_refract(vx, vy, vz, 0.f, 0.f, -1.f, 1.f, P.n1, R) || !obd_pass(ex, u[0], R, wt)) return 0.f;
     // lens 1, convex face (glass -> air): far side of the sphere about (0, 0, c_lens1)
     {
@@ -170,40 +184,110 @@ __device__ float obd_trace(const ObdParams& P, const ObdStep& S, float x, float 
     return (x * x + y * y < P.pd_r_sq) ? wt : 0.f;
 }
 
+// A step's lens-1 aperture test: the first test of obd_trace, computed the same way.
+__device__ __forceinline__ bool obd_enters(const ObdParams& P, const ObdStep& S, float x, float y, float vx, float vy,
+                                           float ivz)
+{
+    obd_lens1_entry(S, x, y, vx, vy, ivz);
+    return x * x + y * y <= P.outer1_sq;
+}
+
+constexpr int kObdUnroll = 4;    // focus steps tested per pass (independent, for latency hiding)
+constexpr int kObdQueue = 512;   // per-wave queue of (lane, step) pairs: < 64 + 64 * kObdUnroll held
+
+// One lane holds one exit packet for the whole scan. Each step, the wave tests all its
+// packets against lens 1's aperture, where about 93% of traces end, and appends the
+// (lane, step) pairs that enter to a per-wave LDS queue. Whenever the queue holds 64
+// pairs, every lane traces one of them through the rest of the receiver, reading the
+// packet's state from its owner lane. Full traces thus run with the wave full, instead of
+// once per step for a handful of lanes. Each (packet, step) is traced by the same
+// arithmetic as before; only the summation order of I(Z) changes.
 __global__ void __launch_bounds__(256) obd_kernel(const ObdParams P)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     ObdStep* steps = reinterpret_cast<ObdStep*>(smem);
     double* hist = reinterpret_cast<double*>(smem + (size_t)P.n_steps * sizeof(ObdStep));
+    uint16_t* queues = reinterpret_cast<uint16_t*>(smem + (size_t)P.n_steps * (sizeof(ObdStep) + sizeof(double)));
     for (int32_t i = threadIdx.x; i < P.n_steps; i += blockDim.x) {
         steps[i] = P.steps[i];
         hist[i] = 0.0;
     }
     __syncthreads();
 
+    const uint32_t lane = threadIdx.x & 63u;
+    const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+    uint16_t* q = queues + (threadIdx.x >> 6) * kObdQueue;
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
-    for (unsigned long long j = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; j < P.n; j += stride) {
-        float x = P.p[j], y = P.p[P.stride + j], z = P.p[2 * P.stride + j];
-        float vx = P.v[j], vy = P.v[P.stride + j], vz = P.v[2 * P.stride + j];
-        const float w = P.w[j];
-        if (P.side != 0.f) {
-            // side view (_classes.py:504-508): axes x and z exchanged, shifted by half the side
-            const float t = x; x = z - 0.5f * P.side; z = t + 0.5f * P.side;
-            const float tv = vx; vx = vz; vz = tv;
-        }
-        (void)z;  // the trace starts on the sample plane, p_z = 0 (:515)
-        if (!(P.inversion ? vz > 0.f : vz < 0.f)) continue;
-        if (P.inversion) { y = -y; vz = -vz; vy = -vy; }
-        for (int32_t s = 0; s < P.n_steps; ++s) {
-            const ObdStep S = steps[s];
-            float u[4] = {0.f, 0.f, 0.f, 0.f};
-            if (!P.expect) {
-                const uint4 r = philox4x32_10(make_uint4((uint32_t)s, kTagObd, (uint32_t)j, (uint32_t)(j >> 32)),
-                                              P.seed, P.stream);
-                u[0] = u01<float>(r.x); u[1] = u01<float>(r.y); u[2] = u01<float>(r.z); u[3] = u01<float>(r.w);
+    // the wave's first packet; the loop bound is wave-uniform, so ballots see every lane
+    for (unsigned long long j0 = (unsigned long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); j0 < P.n;
+         j0 += stride) {
+        const unsigned long long j = j0 + lane;
+        bool valid = j < P.n;
+        float x = 0.f, y = 0.f, vx = 0.f, vy = 0.f, vz = 1.f, w = 0.f;
+        if (valid) {
+            x = P.p[j]; y = P.p[P.stride + j];
+            const float z = P.p[2 * P.stride + j];
+            vx = P.v[j]; vy = P.v[P.stride + j]; vz = P.v[2 * P.stride + j];
+            w = P.w[j];
+            if (P.side != 0.f) {
+                // side view (_classes.py:504-508): axes x and z exchanged, shifted by half the side
+                x = z - 0.5f * P.side;
+                const float tv = vx; vx = vz; vz = tv;
             }
-            const float d = obd_trace(P, S, x + S.x_shift, y, vx, vy, vz, w, u);
-            if (d > 0.f) atomicAdd(&hist[s], (double)d);
+            // the trace starts on the sample plane, p_z = 0 (:515)
+            valid = P.inversion ? vz > 0.f : vz < 0.f;
+            if (P.inversion) { y = -y; vz = -vz; vy = -vy; }
+            const float inv = __builtin_amdgcn_rsqf(vx * vx + vy * vy + vz * vz);
+            vx *= inv; vy *= inv; vz *= inv;
+        }
+        const float ivz = 1.f / vz;
+        uint32_t head = 0, tail = 0;   // wave-uniform queue positions
+        for (int32_t s0 = 0; s0 <= P.n_steps; s0 += kObdUnroll) {
+            if (s0 < P.n_steps) {
+                bool enters[kObdUnroll];
+#pragma unroll
+                for (int u = 0; u < kObdUnroll; ++u) {
+                    const int32_t s = s0 + u;
+                    enters[u] = false;
+                    if (valid && s < P.n_steps) {
+                        const ObdStep& S = steps[s];
+                        enters[u] = obd_enters(P, S, x + S.x_shift, y, vx, vy, ivz);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kObdUnroll; ++u) {
+                    const unsigned long long m = __ballot(enters[u]);
+                    if (enters[u])
+                        q[(tail + (uint32_t)__popcll(m & lt_mask)) & (kObdQueue - 1)] =
+                            (uint16_t)(((s0 + u) << 6) | (int32_t)lane);
+                    tail += (uint32_t)__popcll(m);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // trace 64 queued pairs at a time; after the last step, the remainder
+            const bool last = s0 + kObdUnroll > P.n_steps;
+            while (tail - head >= 64u || (last && tail != head)) {
+                const uint32_t take = tail - head < 64u ? tail - head : 64u;
+                const bool mine = lane < take;
+                const uint32_t e = mine ? (uint32_t)q[(head + lane) & (kObdQueue - 1)] : 0u;
+                const int src = (int)(e & 63u);
+                const int32_t es = (int32_t)(e >> 6);
+                const float tx = __shfl(x, src), ty = __shfl(y, src);
+                const float tvx = __shfl(vx, src), tvy = __shfl(vy, src), tvz = __shfl(vz, src);
+                const float tivz = __shfl(ivz, src), tw = __shfl(w, src);
+                if (mine) {
+                    const ObdStep& S = steps[es];
+                    const float d = obd_trace(P, S, tx + S.x_shift, ty, tvx, tvy, tvz, tivz, tw, es, j0 + (uint32_t)src);
+                    if (d > 0.f) atomicAdd(&hist[es], (double)d);
+                }
+                head += take;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+            if (last) break;
         }
     }
     __syncthreads();

@@ -834,7 +834,9 @@ int pmo_obd_trace(const pmo_obd* cfg, const float* p, const float* v, const floa
     P.out = intensity;
 
     const int bs = 256;
-    const size_t lds = (size_t)n_steps * (sizeof(pmo::ObdStep) + sizeof(double));
+    // step table | FP64 I(Z) bins | one queue of uint16 (lane, step) pairs per wave
+    const size_t lds = (size_t)n_steps * (sizeof(pmo::ObdStep) + sizeof(double)) +
+                       (size_t)(bs / 64) * pmo::kObdQueue * sizeof(uint16_t);
     unsigned long long blocks = ((unsigned long long)n + bs - 1) / bs;
     const unsigned long long cap = (unsigned long long)d.n_cu * 8ull;
     if (blocks > cap) blocks = cap;

@@ -216,9 +216,19 @@ class OBD:
         return self._launch(base, base + 3 * n * 4, base + 6 * n * 4, n, n, seed, stream, expect, hip_stream,
                             keep=(es,))
 
+    def _device_steps(self):
+        """The step table on the device, rebuilt only when the parameters change (a scan
+        repeated over several walks reuses it without host work or a copy)."""
+        key = repr(sorted((k, np.asarray(v).tolist()) for k, v in self.params.items()))
+        cached = getattr(self, "_steps_cache", None)
+        if cached is None or cached[0] != key:
+            Z, table = self.step_table()
+            cached = (key, Z, torch.from_numpy(table).to(self.device))
+            self._steps_cache = cached
+        return cached[1], cached[2]
+
     def _launch(self, pp, vp, wp, n, stride, seed, stream, expect, hip_stream, keep):
-        Z, table = self.step_table()
-        steps = torch.from_numpy(table).to(self.device)
+        Z, steps = self._device_steps()
         out = torch.zeros(Z.size, dtype=torch.float64, device=self.device)
         cfg = self._config(seed, stream, expect)
         lib = _abi.lib()
