@@ -19,9 +19,11 @@
  *
  * Conventions: all pointers marked "device" are HIP device pointers owned by the caller;
  * the library never frees them and keeps no device state between calls except its code
- * objects. Tally buffers are accumulated into (caller zeroes them). The library allocates
- * nothing: per-call scratch lives in the caller-provided workspace of
- * pmo_workspace_bytes() bytes, so every entry point may be captured into a HIP graph.
+ * objects. Tally buffers are accumulated into (caller zeroes them). The walk entry points
+ * allocate nothing: per-call scratch lives in the caller-provided workspace of
+ * pmo_workspace_bytes() bytes, so they may be captured into a HIP graph. The one exception
+ * is pmo_turing_create, whose hipFFT plans hold hipFFT's own tables until
+ * pmo_turing_destroy. tests/c/walk_slab.c is a complete C caller.
  * A `stream` argument is a hipStream_t (NULL = the default stream).
  * Return value: PMO_OK or a negative PMO_ERR_* code; pmo_last_error() gives the message
  * (thread-local).

@@ -114,3 +114,17 @@ def test_philox_argument_errors():
     assert L.pmo_philox(None, 0, 0, None, -1, None) == _abi.PMO_ERR_INVALID
     assert L.pmo_philox(None, 0, 0, None, 0, None) == _abi.PMO_OK
     assert L.pmo_philox(None, 0, 0, None, 4, None) == _abi.PMO_ERR_INVALID
+
+
+def test_c_example_builds_against_the_header(tmp_path):
+    """tests/c/walk_slab.c uses include/pmo.h and libpmo.so only (C11, no C++ or torch): it
+    compiles with gcc and links against every symbol it calls."""
+    import subprocess
+    root = ROOT
+    out = tmp_path / "walk_slab"
+    cmd = ["gcc", "-O2", "-std=c11", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__", f"-I{root}/include",
+           "-I/opt/rocm/include", os.path.join(ROOT, "tests", "c", "walk_slab.c"), "-o", str(out),
+           f"-L{root}/pymonteopt_amd", "-lpmo", "-L/opt/rocm/lib", "-lamdhip64"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert out.exists()
