@@ -290,9 +290,17 @@ def virtual_obd_leg(n_packets=10_000_000, seeds=(4, 5)):
                              "start_s": t4 - t3, "save_and_csv_s": t5 - t4, "subject_s": t5 - t0,
                              "walk_photons_per_s": n_packets / m.time_s, "Rd": res["Rd"], "Tt": res["Tt"]})
             del m, u
+        # the script's loop as virtual_obd.run runs it: subject i walks on its own stream while
+        # subject i + 1 is drawn, grown and wrapped
+        np.random.seed(seeds[-1])
+        t0 = time.perf_counter()
+        done = vo.run(3, overlap=True, root=os.path.join(root, "loop"), nPh=n_packets, model_params=dict(vo.model_params),
+                      monte_params=dict(vo.monte_params))
+        loop_s = (time.perf_counter() - t0) / max(1, len(done))
     finally:
         shutil.rmtree(root, ignore_errors=True)
-    return dict(subjects[-1], first_subject_s=subjects[0]["subject_s"], subjects=subjects)
+    return dict(subjects[-1], first_subject_s=subjects[0]["subject_s"], run_overlapped_s_per_subject=loop_s,
+                subjects=subjects)
 
 
 def cpu_baseline_turing(n=128, steps=4):

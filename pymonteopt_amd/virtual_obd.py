@@ -132,18 +132,18 @@ def subject_params(vp, params, nn=0):
     return params
 
 
-def calc_montecalro(vp, iteral, params, path, u, nPh=nPh, device="cuda:0"):
-    """:127-156: walks nPh packets through the subject's bone model with the walk seeded by
-    `iteral`, saves <path>_LID.pkl.bz2 and <path>_info.json, and returns (result, params)."""
+def _subject_model(vp, params, u, nPh, device):
+    """The subject's VoxelTuringModel, built on the host (the layers wrapped around u)."""
     from .voxel import VoxelTuringModel
-    print()
-    print("###################################")
-    print("# %s" % iteral)
     subject_params(vp, params)
     model = VoxelTuringModel(nPh=nPh, model_name="TuringModel", nr=RD_BINS, dr=RD_WIDTH, nl=LINE_BINS,
                              dl=LINE_WIDTH, line_y=LINE_Y, fluence=False, device=device)
     model.set_model(u)
     model.build(**{k: v for k, v in params.items() if k not in _UNVERIFIED_KEYS})
+    return model
+
+
+def _walk_and_save(model, iteral, path):
     start = time.time()
     model = model.start(iteral)
     print("%s sec" % (time.time() - start))
@@ -151,7 +151,17 @@ def calc_montecalro(vp, iteral, params, path, u, nPh=nPh, device="cuda:0"):
     print("Save -> %s" % path)
     os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
     model.save_result(path, coment="for machine learning")
-    res = model.get_result()
+    return model.get_result()
+
+
+def calc_montecalro(vp, iteral, params, path, u, nPh=nPh, device="cuda:0"):
+    """:127-156: walks nPh packets through the subject's bone model with the walk seeded by
+    `iteral`, saves <path>_LID.pkl.bz2 and <path>_info.json, and returns (result, params)."""
+    print()
+    print("###################################")
+    print("# %s" % iteral)
+    model = _subject_model(vp, params, u, nPh, device)
+    res = _walk_and_save(model, iteral, path)
     del model
     gc.collect()
     return res, params
@@ -204,9 +214,8 @@ def calc_ray_tracing(res, monte_params, path, alias_name):
         df.to_csv(path + suffix + ".csv")
 
 
-def calc(iteral, *, root=".", nPh=nPh, device="cuda:0", model_params=model_params, monte_params=monte_params):
-    """:198-237: one subject. Returns the alias name, or None when the draw is invalid
-    (BV/TV or cortical thickness <= 0)."""
+def _draw_subject(iteral):
+    """The subject's parameters (:199-203), or None for an invalid draw."""
     gvp = generate_variable_params()
     range_params = determining_params_range()
     gvp.set_params(range_params)
@@ -216,31 +225,98 @@ def calc(iteral, *, root=".", nPh=nPh, device="cuda:0", model_params=model_param
         print("BV/TV : ", vp["bv_tv"][0])
         print("th_cortical : ", vp["th_cortical"][0])
         return None
+    return vp
+
+
+def _paths(root, alias_name):
+    return tuple(os.path.join(root, d, "") + alias_name for d in ("model_result", "monte_result", "opt_result"))
+
+
+def calc(iteral, *, root=".", nPh=nPh, device="cuda:0", model_params=model_params, monte_params=monte_params):
+    """:198-237: one subject. Returns the alias name, or None when the draw is invalid
+    (BV/TV or cortical thickness <= 0)."""
+    vp = _draw_subject(iteral)
+    if vp is None:
+        return None
     alias_name = "-".join((str(datetime.datetime.now().isoformat()).split(".")[0]).split(":")) + "_it" + f"{iteral:04}"
     print("### iteral number ", iteral)
     print("Alias name: ", alias_name)
-    model_path = os.path.join(root, "model_result", "")
-    monte_path = os.path.join(root, "monte_result", "")
-    opt_path = os.path.join(root, "opt_result", "")
-    u, bv_tv = generate_bone_model(vp["bv_tv"][0], model_path + alias_name + "_dicom", model_params, device)
+    model_path, monte_path, opt_path = _paths(root, alias_name)
+    u, bv_tv = generate_bone_model(vp["bv_tv"][0], model_path + "_dicom", model_params, device)
     print("it: ", iteral, ", change bvtv: ", vp["bv_tv"][0], "-->", bv_tv)
     vp["bv_tv"][0] = bv_tv
-    res, params_ = calc_montecalro(vp, iteral, monte_params, monte_path + alias_name, u, nPh=nPh, device=device)
+    res, params_ = calc_montecalro(vp, iteral, monte_params, monte_path, u, nPh=nPh, device=device)
     print("###### end monte calro in it: ", iteral)
-    calc_ray_tracing(res, params_, opt_path + alias_name, alias_name)
+    calc_ray_tracing(res, params_, opt_path, alias_name)
     print("")
     print("############### End %s it ###################" % iteral)
     print("")
     return alias_name
 
 
-def run(repetitions=repetitions, **kwargs):
-    """The script's main loop (:241-247)."""
-    for iteral in range(repetitions):
-        calc(iteral, **kwargs)
+def _prepare(iteral, root, nPh, device, model_params, monte_params):
+    """The host side of one subject and its bone growth: draws, Turing volume, wrapped model."""
+    vp = _draw_subject(iteral)
+    if vp is None:
+        return None
+    alias_name = "-".join((str(datetime.datetime.now().isoformat()).split(".")[0]).split(":")) + "_it" + f"{iteral:04}"
+    print("### iteral number ", iteral, " alias ", alias_name)
+    model_path, monte_path, opt_path = _paths(root, alias_name)
+    u, bv_tv = generate_bone_model(vp["bv_tv"][0], model_path + "_dicom", model_params, device)
+    vp["bv_tv"][0] = bv_tv
+    params = dict(monte_params)
+    model = _subject_model(vp, params, u, nPh, device)
+    return iteral, alias_name, model, params, monte_path, opt_path
+
+
+def _finish(prep, stream):
+    """The device side of one subject on `stream`: upload, walk, result files, detector CSVs."""
+    import torch
+    iteral, alias_name, model, params, monte_path, opt_path = prep
+    with torch.cuda.stream(stream):
+        res = _walk_and_save(model, iteral, monte_path)
+        calc_ray_tracing(res, params, opt_path, alias_name)
+    print("############### End %s it ###################" % iteral)
+    return alias_name
+
+
+def run(repetitions=repetitions, *, overlap=True, root=".", nPh=nPh, device="cuda:0", model_params=model_params,
+        monte_params=monte_params):
+    """The script's main loop (:241-247). Returns the alias names of the valid subjects.
+
+    With overlap (the default), subject i walks on its own HIP stream in a worker thread
+    while the main thread draws subject i + 1, grows its bone and wraps its layers. The
+    draws from np.random keep the script's order, and each walk is seeded by its
+    iteration number, so the files are those of the sequential loop (overlap=False, which
+    calls `calc` as the script does). The subjects' parameters go to copies of
+    monte_params instead of the shared dict."""
+    aliases = []
+    if not overlap:
+        for iteral in range(repetitions):
+            a = calc(iteral, root=root, nPh=nPh, device=device, model_params=model_params, monte_params=monte_params)
+            if a is not None:
+                aliases.append(a)
+    else:
+        from concurrent.futures import ThreadPoolExecutor
+
+        import torch
+        stream = torch.cuda.Stream(device=device)
+        with ThreadPoolExecutor(max_workers=1) as pool:
+            pending = None
+            for iteral in range(repetitions):
+                prep = _prepare(iteral, root, nPh, device, model_params, monte_params)
+                if pending is not None:
+                    aliases.append(pending.result())    # at most one subject in flight
+                    pending = None
+                if prep is not None:
+                    pending = pool.submit(_finish, prep, stream)
+                del prep
+            if pending is not None:
+                aliases.append(pending.result())
     print()
     print("######################")
     print(datetime.datetime.now())
+    return aliases
 
 
 if __name__ == "__main__":

@@ -79,3 +79,26 @@ def test_device_detectors_match_the_exit_dump(tmp_path):
         assert np.array_equal(ad, ah), name
         assert np.allclose(yd, yh, rtol=1e-5, atol=1e-12), (name, np.abs(yd - yh).max())
         assert yd.sum() > 0, name
+
+
+def test_overlapped_run_writes_the_sequential_files(tmp_path):
+    """run(overlap=True) walks subject i on its own stream while subject i + 1 is prepared;
+    with the same np.random seed it writes the curves the sequential loop writes."""
+    from pymonteopt_amd import virtual_obd as vo
+    out = {}
+    for overlap in (False, True):
+        mp, mo = _params()
+        np.random.seed(4)
+        root = tmp_path / ("ov" if overlap else "seq")
+        aliases = vo.run(3, overlap=overlap, root=str(root), nPh=100000, model_params=mp, monte_params=mo)
+        assert len(aliases) == 3
+        curves = {}
+        for a in aliases:
+            it = a.split("_it")[-1]
+            for suffix in ("_B", "_F", "_L"):
+                curves[it + suffix] = pd.read_csv(root / "opt_result" / (a + suffix + ".csv"), index_col=0).iloc[:, 0].values
+        out[overlap] = curves
+    assert sorted(out[False]) == sorted(out[True])
+    for k, v in out[False].items():
+        assert np.allclose(out[True][k], v, rtol=1e-12, atol=0), k
+        assert v.sum() > 0, k
