@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define PMO_ABI_VERSION 3
+#define PMO_ABI_VERSION 4
 
 /* ---- error codes ---------------------------------------------------------------- */
 #define PMO_OK 0
@@ -51,6 +51,9 @@ extern "C" {
 #define PMO_BEAM_PENCIL 0 /* all packets start at the centre of the launch voxel's top face  */
 #define PMO_BEAM_FLAT 1   /* (x, y) uniform in a disc of radius `radius` (mm)               */
 #define PMO_BEAM_GAUSS 2  /* TEM00: x, y ~ N(0, radius/2), radius = beam waist w (mm)       */
+#define PMO_BEAM_STATES 3 /* per-packet launch states from the caller (pmo_source.add/p/v/w),
+                             the arrays the reference kernels take (pymopt/voxel/_kernel.py:23-28,
+                             pymopt/voxel_gpu/_cukernel.py:27-31); pmo_walk only             */
 
 /* ---- run flags ------------------------------------------------------------------- */
 #define PMO_F_FP64 (1u << 0)           /* walk in double precision (checking build)        */
@@ -109,6 +112,16 @@ typedef struct pmo_source {
     float radius;          /* disc radius / Gaussian waist (mm), unused for the pencil beam    */
     float dir[3];          /* launch direction, unit vector (reference _set_inital_vector)     */
     float w0;              /* launch weight, 1 - Rsp (reference _set_inital_w)                 */
+    /* PMO_BEAM_STATES only (else NULL/0): device arrays indexed by packet id - first_packet,
+       component c of packet i at [c * states_stride + i]. add is the voxel address, p the
+       position in the voxel's frame ([-pitch/2, pitch/2] per axis; outside values are clamped),
+       v the direction, w the weight. An address outside the volume leaves at once. ix, iy, iz
+       still give the centre of the LDS tally window (and must be an interior voxel).          */
+    const int32_t* add;    /* [3][states_stride]                                                */
+    const float* p;        /* [3][states_stride]                                                */
+    const float* v;        /* [3][states_stride]                                                */
+    const float* w;        /* [states_stride]                                                   */
+    uint64_t states_stride; /* >= n_packets                                                     */
 } pmo_source;
 
 typedef struct pmo_run {

@@ -38,11 +38,18 @@ typedef struct og_volume {
 } og_volume;
 
 typedef struct og_source {
-    int32_t kind;          /* 0 pencil, 1 flat disc, 2 gaussian                      */
+    int32_t kind;          /* 0 pencil, 1 flat disc, 2 gaussian, 3 per-packet states */
     int32_t ix, iy, iz;
     double radius;
     double dir[3];
     double w0;
+    /* kind 3: the reference kernel's launch arrays (pymopt/voxel/_kernel.py:23-28), packet
+       i's component c at [c * states_stride + i], i = packet id - first_packet */
+    const int32_t* add;
+    const float* p;
+    const float* v;
+    const float* w;
+    uint64_t states_stride;
 } og_source;
 
 typedef struct og_run {

@@ -31,7 +31,9 @@ class OgVolume(C.Structure):
 
 class OgSource(C.Structure):
     _fields_ = [("kind", C.c_int32), ("ix", C.c_int32), ("iy", C.c_int32), ("iz", C.c_int32),
-                ("radius", C.c_double), ("dir", C.c_double * 3), ("w0", C.c_double)]
+                ("radius", C.c_double), ("dir", C.c_double * 3), ("w0", C.c_double),
+                ("add", C.c_void_p), ("p", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p),
+                ("states_stride", C.c_uint64)]
 
 
 class OgRun(C.Structure):
@@ -158,6 +160,7 @@ def _volume(labels, pitch, mua, mus, g, n):
 
 
 def _source(src):
+    """The source struct, and the arrays it points to (kept alive by the caller)."""
     s = OgSource()
     s.kind = int(src.get("kind", 0))
     s.ix, s.iy, s.iz = (int(x) for x in src["voxel"])
@@ -166,7 +169,18 @@ def _source(src):
     for i in range(3):
         s.dir[i] = float(d[i])
     s.w0 = float(src["w0"])
-    return s
+    keep = ()
+    if src.get("states") is not None:
+        # per-packet launch state (kind 3): add, p, v as (3, N), w as (N,)
+        add, p, v, w = src["states"]
+        add = np.ascontiguousarray(add, np.int32)
+        p, v = np.ascontiguousarray(p, np.float32), np.ascontiguousarray(v, np.float32)
+        w = np.ascontiguousarray(w, np.float32)
+        s.kind = 3
+        s.add, s.p, s.v, s.w = (a.ctypes.data for a in (add, p, v, w))
+        s.states_stride = w.shape[0]
+        keep = (add, p, v, w)
+    return s, keep
 
 
 def walk(labels, pitch, mua, mus, g, n, source, n_packets, seed, *, stream=0, first_packet=0,
@@ -175,7 +189,7 @@ def walk(labels, pitch, mua, mus, g, n, source, n_packets, seed, *, stream=0, fi
          threads=0):
     """Run the oracle walk; returns the raw tally dictionary (unnormalised sums)."""
     vol, keep = _volume(labels, pitch, mua, mus, g, n)
-    src = _source(source)
+    src, keep_src = _source(source)
     run = OgRun(int(n_packets), int(first_packet), int(seed) & 0xFFFFFFFF, int(stream) & 0xFFFFFFFF,
                 float(wth), float(roulette_m), int(flags), int(threads))
     t = OgTallies()
@@ -243,7 +257,7 @@ def trace(labels, pitch, mua, mus, g, n, source, pid, seed, *, stream=0, wth=1e-
           flags=0, cap=100000):
     """Event trace of one packet (FP64 oracle)."""
     vol, keep = _volume(labels, pitch, mua, mus, g, n)
-    src = _source(source)
+    src, keep_src = _source(source)
     run = OgRun(1, int(pid), int(seed), int(stream), float(wth), float(roulette_m), int(flags), 1)
     ev = (OgEvent * cap)()
     m = lib().og_trace_f64(C.byref(vol), C.byref(src), C.byref(run), int(pid), ev, int(cap))

@@ -198,7 +198,16 @@ static int FN(packet)(const og_volume* V, const og_source* S, const og_run* R, c
     const int want_path = (C && C->rd_wl) ? 1 : 0;
     if (want_path) for (int j = 0; j < V->n_labels; j++) { Lj[j] = 0; nj[j] = 0; }
 
-    if (S->kind == 1 || S->kind == 2) {                  /* flat disc / TEM00 lateral position */
+    if (S->kind == 3) {                                  /* the caller's launch state */
+        const uint64_t sd = S->states_stride;
+        for (int c = 0; c < 3; c++) {
+            a[c] = S->add[c * sd + slot];
+            p[c] = (REAL)S->p[c * sd + slot];
+            if (p[c] < -half) p[c] = -half;
+            if (p[c] > half) p[c] = half;
+            v[c] = (REAL)S->v[c * sd + slot];
+        }
+    } else if (S->kind == 1 || S->kind == 2) {           /* flat disc / TEM00 lateral position */
         uint32_t c[4] = {0u, 2u, (uint32_t)pid, (uint32_t)(pid >> 32)}, k[2] = {R->seed, R->stream}, o[4];
         og_philox4x32_10(c, k, o);
         REAL u0 = (REAL)og_u01(o[0]), u1 = (REAL)og_u01(o[1]);
@@ -218,7 +227,7 @@ static int FN(packet)(const og_volume* V, const og_source* S, const og_run* R, c
         if (p[1] < -half) p[1] = -half;
         if (p[1] > half) p[1] = half;
     }
-    REAL w = (REAL)S->w0;
+    REAL w = S->kind == 3 ? (REAL)S->w[slot] : (REAL)S->w0;
     if (C) { C->scal[6] += (double)w; C->ev[0]++; }
     if (trace && ntr < cap) {
         og_event* e = &trace[ntr++];
@@ -227,7 +236,16 @@ static int FN(packet)(const og_volume* V, const og_source* S, const og_run* R, c
         e->w = w;
     }
 
-    int lab = V->labels[FN(lin)(V, a[0], a[1], a[2])];     /* :44 */
+    /* :44; a caller's address outside the volume leaves at once (as in libpmo) */
+    const int inside = a[0] >= 0 && a[0] < V->nx && a[1] >= 0 && a[1] < V->ny && a[2] >= 0 && a[2] < V->nz;
+    int lab = inside ? V->labels[FN(lin)(V, a[0], a[1], a[2])] : end;
+    if (lab >= end) {                    /* launched in an end voxel: leaves along v_z at once */
+        if (C) {
+            C->ev[6]++;
+            FN(record_exit)(V, T, C, slot, R->n_packets, a, p, v, w, 2, v[2] > 0 ? 1 : -1, Lj, nj);
+        }
+        return ntr;
+    }
     REAL ni = (REAL)V->n[lab];                              /* :54 */
     REAL mt = (REAL)V->mua[lab] + (REAL)V->mus[lab];        /* :56 */
     uint32_t inter = 0, fres = 0;

@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpmo.so")
 
-ABI_VERSION = 3          # PMO_ABI_VERSION of include/pmo.h
+ABI_VERSION = 4          # PMO_ABI_VERSION of include/pmo.h
 
 PMO_OK = 0
 PMO_ERR_INVALID = -1
@@ -22,6 +22,7 @@ PMO_ERR_UNSUPPORTED = -4
 PMO_BEAM_PENCIL = 0
 PMO_BEAM_FLAT = 1
 PMO_BEAM_GAUSS = 2
+PMO_BEAM_STATES = 3
 
 PMO_F_FP64 = 1 << 0
 PMO_F_NORMAL_FRESNEL = 1 << 1
@@ -44,7 +45,9 @@ class PmoVolume(C.Structure):
 
 class PmoSource(C.Structure):
     _fields_ = [("kind", C.c_int32), ("ix", C.c_int32), ("iy", C.c_int32), ("iz", C.c_int32),
-                ("radius", C.c_float), ("dir", C.c_float * 3), ("w0", C.c_float)]
+                ("radius", C.c_float), ("dir", C.c_float * 3), ("w0", C.c_float),
+                ("add", C.c_void_p), ("p", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p),
+                ("states_stride", C.c_uint64)]
 
 
 class PmoRun(C.Structure):

@@ -7,9 +7,9 @@ the model prepared (add, p, v, w of every packet), runs libpmo's walk on the GPU
 per-packet exit states that the model's own _encooder / getRdTtRate
 (pymopt/voxel/_classes.py:261-293) turn into the same results.
 
-Launch states the device walk generates itself are accepted: all packets identical
-(pencil beam, optionally oblique). Per-packet launch arrays of other shapes (the reference's
-Gaussian beam) are rejected; use the model API's w_beam instead.
+Any launch state is accepted. When all packets share one (the pencil beam, optionally
+oblique) the device generates it; otherwise the arrays themselves are handed to the walk
+(PMO_BEAM_STATES), as with the reference's Gaussian beam.
 """
 from __future__ import annotations
 
@@ -32,11 +32,7 @@ def vmc_kernel(add, p, v, w, ma, ms, n, g, voxel_model, l, nPh, end_point, rand_
     N = int(nPh)
     if add.shape != (3, N) or p.shape != (3, N) or v.shape != (3, N) or w.shape != (N,):
         raise ValueError("add, p, v must be (3, nPh) and w (nPh,)")
-    for arr in (add, p, v):
-        if not np.all(arr == arr[:, :1]):
-            raise ValueError("per-packet launch states differ; only identical launches are generated on the device")
-    if not np.all(w == w[0]):
-        raise ValueError("per-packet launch weights differ")
+    identical = all(np.all(arr == arr[:, :1]) for arr in (add, p, v)) and np.all(w == w[0])
     vm = np.ascontiguousarray(voxel_model).astype(np.uint8)
     end = int(end_point)
     nl = int(np.asarray(ma).size)
@@ -47,10 +43,14 @@ def vmc_kernel(add, p, v, w, ma, ms, n, g, voxel_model, l, nPh, end_point, rand_
     opt[:nl, 1] = np.asarray(ms, np.float32)
     opt[:nl, 2] = np.asarray(g, np.float32)
     opt[:, 3] = np.asarray(n, np.float32)[: nl + 1]
-    if not np.isclose(float(p[2, 0]), -float(l) / 2, rtol=1e-5) or p[0, 0] != 0 or p[1, 0] != 0:
-        raise ValueError("launch position must be the centre of the voxel's top face (0, 0, -l/2)")
-    src = Source(voxel=tuple(int(x) for x in add[:, 0]), w0=float(w[0]),
-                 dir=tuple(float(x) for x in v[:, 0]))
+    centre = (vm.shape[0] - 1) // 2, (vm.shape[1] - 1) // 2, 1
+    on_face = np.isclose(float(p[2, 0]), -float(l) / 2, rtol=1e-5) and p[0, 0] == 0 and p[1, 0] == 0
+    if identical and on_face:
+        src = Source(voxel=tuple(int(x) for x in add[:, 0]), w0=float(w[0]),
+                     dir=tuple(float(x) for x in v[:, 0]))
+    else:
+        # the caller's arrays; the LDS tally window sits at the top-face centre
+        src = Source(voxel=centre, w0=1.0, states=(add.astype(np.int32), p, v, w))
     seed = np.random.randint(1e5) if rand_seed is None else int(rand_seed)
     wk = Walker(vm, float(l), opt)
     tal = wk.tallies(TallySpec(absorbed=False, exit_state=True), N)

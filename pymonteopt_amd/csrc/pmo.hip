@@ -141,14 +141,22 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     if (vol->n_labels < 1 || vol->n_labels > 255)
         return fail(PMO_ERR_LABEL, "n_labels = %d outside [1, 255]", vol->n_labels);
     if (!(vol->pitch > 0.f)) return fail(PMO_ERR_INVALID, "pitch must be > 0");
-    if (src->kind < PMO_BEAM_PENCIL || src->kind > PMO_BEAM_GAUSS)
+    if (src->kind < PMO_BEAM_PENCIL || src->kind > PMO_BEAM_STATES)
         return fail(PMO_ERR_INVALID, "unknown beam kind %d", src->kind);
     if (src->ix < 1 || src->ix > vol->nx - 2 || src->iy < 1 || src->iy > vol->ny - 2 || src->iz < 1 ||
         src->iz > vol->nz - 2)
         return fail(PMO_ERR_INVALID, "launch voxel (%d, %d, %d) is not inside the volume interior", src->ix,
                     src->iy, src->iz);
-    const float dn = src->dir[0] * src->dir[0] + src->dir[1] * src->dir[1] + src->dir[2] * src->dir[2];
-    if (!(dn > 0.999f && dn < 1.001f)) return fail(PMO_ERR_INVALID, "launch direction is not a unit vector");
+    if (src->kind == PMO_BEAM_STATES) {
+        if (!src->add || !src->p || !src->v || !src->w)
+            return fail(PMO_ERR_INVALID, "per-packet launch states need add, p, v and w");
+        if (src->states_stride < run->n_packets)
+            return fail(PMO_ERR_INVALID, "states_stride %llu < n_packets %llu",
+                        (unsigned long long)src->states_stride, (unsigned long long)run->n_packets);
+    } else {
+        const float dn = src->dir[0] * src->dir[0] + src->dir[1] * src->dir[1] + src->dir[2] * src->dir[2];
+        if (!(dn > 0.999f && dn < 1.001f)) return fail(PMO_ERR_INVALID, "launch direction is not a unit vector");
+    }
     if (!(run->roulette_m >= 1.f)) return fail(PMO_ERR_INVALID, "roulette_m must be >= 1");
     if (run->n_packets == 0) return PMO_OK;
     const bool wide = run->n_packets >= kWideRunPackets && tal->absorbed != nullptr;
@@ -179,6 +187,10 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     P.src_kind = src->kind; P.ix = src->ix; P.iy = src->iy; P.iz = src->iz;
     P.radius = src->radius; P.dirx = src->dir[0]; P.diry = src->dir[1]; P.dirz = src->dir[2];
     P.w0 = src->w0;
+    if (src->kind == PMO_BEAM_STATES) {
+        P.st_add = src->add; P.st_p = src->p; P.st_v = src->v; P.st_w = src->w;
+        P.st_stride = src->states_stride;
+    }
     P.n_packets = run->n_packets; P.first_packet = run->first_packet;
     P.seed = run->seed; P.stream = run->stream;
     P.wth = run->wth; P.roulette_m = run->roulette_m; P.inv_m = 1.f / run->roulette_m;

@@ -36,6 +36,7 @@ template <> struct Mth<float> {
     static __device__ __forceinline__ float fabs(float x) { return __builtin_fabsf(x); }
     static __device__ __forceinline__ float copysign(float a, float b) { return __builtin_copysignf(a, b); }
     static __device__ __forceinline__ float fmax(float a, float b) { return __builtin_fmaxf(a, b); }
+    static __device__ __forceinline__ float fmin(float a, float b) { return __builtin_fminf(a, b); }
     static __device__ __forceinline__ float floor(float x) { return __builtin_floorf(x); }
 };
 
@@ -52,6 +53,7 @@ template <> struct Mth<double> {
     static __device__ __forceinline__ double fabs(double x) { return __builtin_fabs(x); }
     static __device__ __forceinline__ double copysign(double a, double b) { return __builtin_copysign(a, b); }
     static __device__ __forceinline__ double fmax(double a, double b) { return __builtin_fmax(a, b); }
+    static __device__ __forceinline__ double fmin(double a, double b) { return __builtin_fmin(a, b); }
     static __device__ __forceinline__ double floor(double x) { return __builtin_floor(x); }
 };
 
@@ -83,6 +85,12 @@ struct WalkParams {
     // source
     int32_t src_kind, ix, iy, iz;
     float radius, dirx, diry, dirz, w0;
+    // per-packet launch states (src_kind 3): [3][st_stride] add, p, v and [st_stride] w
+    const int32_t* st_add;
+    const float* st_p;
+    const float* st_v;
+    const float* st_w;
+    unsigned long long st_stride;
     // run
     unsigned long long n_packets, first_packet;
     uint32_t seed, stream;
@@ -362,14 +370,25 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
                     const uint32_t plo = (uint32_t)pid, phi = (uint32_t)(pid >> 32);
                     ax = P.ix; ay = P.iy; az = P.iz;
                     px = 0; py = 0; pz = -half;
-                    if (P.src_kind != 0)
+                    vx = (R)P.dirx; vy = (R)P.diry; vz = (R)P.dirz;
+                    w = (R)P.w0;
+                    if (P.src_kind == 3) {
+                        // the caller's launch state (the reference kernels' add, p, v, w)
+                        const unsigned long long q = pid - P.first_packet, sd = P.st_stride;
+                        ax = P.st_add[q]; ay = P.st_add[sd + q]; az = P.st_add[2 * sd + q];
+                        px = M::fmin(M::fmax((R)P.st_p[q], -half), half);
+                        py = M::fmin(M::fmax((R)P.st_p[sd + q], -half), half);
+                        pz = M::fmin(M::fmax((R)P.st_p[2 * sd + q], -half), half);
+                        vx = (R)P.st_v[q]; vy = (R)P.st_v[sd + q]; vz = (R)P.st_v[2 * sd + q];
+                        w = (R)P.st_w[q];
+                    } else if (P.src_kind != 0) {
                         launch_xy<R>(P.src_kind, P.radius, pitch, half, plo, phi, seed, skey, (int)P.nx,
                                      (int)P.ny, ax, ay, px, py);
-                    const uint32_t vi = ((uint32_t)ax * P.ny + (uint32_t)ay) * P.nz + (uint32_t)az;
-                    vx = (R)P.dirx; vy = (R)P.diry; vz = (R)P.dirz;
+                    }
                     ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
-                    w = (R)P.w0;
-                    uint32_t l0 = P.labels[vi];
+                    uint32_t l0 = end;   // an address outside the volume leaves at once
+                    if ((uint32_t)ax < P.nx && (uint32_t)ay < P.ny && (uint32_t)az < P.nz)
+                        l0 = P.labels[((uint32_t)ax * P.ny + (uint32_t)ay) * P.nz + (uint32_t)az];
                     lab = l0 < end ? l0 : end;
                     ni = tab[lab].n;
                     mt = tab[lab].mt;

@@ -23,10 +23,16 @@ class Source:
     kind: int = _abi.PMO_BEAM_PENCIL
     radius: float = 0.0     # flat-disc radius / Gaussian waist (mm)
     dir: tuple = (0.0, 0.0, 1.0)
+    # per-packet launch state (add, p, v, w) of the reference kernels: add, p, v (3, N) and
+    # w (N,), host arrays or device tensors; `voxel` then only centres the LDS tally window
+    states: tuple = None
 
     def as_dict(self):
-        return dict(kind=self.kind, voxel=tuple(int(x) for x in self.voxel), w0=float(self.w0),
-                    radius=float(self.radius), dir=tuple(float(x) for x in self.dir))
+        d = dict(kind=self.kind, voxel=tuple(int(x) for x in self.voxel), w0=float(self.w0),
+                 radius=float(self.radius), dir=tuple(float(x) for x in self.dir))
+        if self.states is not None:
+            d["states"] = tuple(x.cpu().numpy() if hasattr(x, "cpu") else x for x in self.states)
+        return d
 
 
 def center_add(length: int) -> int:

@@ -160,6 +160,20 @@ class Walker:
         v.optics = _ptr(self.optics)
         return v
 
+    def _states(self, states, n):
+        """Device copies (int32 add, float32 p, v, w) of a per-packet launch state, checked
+        against n packets."""
+        add, p, v, w = states
+        dev = dict(device=self.device)
+        out = (torch.as_tensor(add).to(dtype=torch.int32, **dev).contiguous(),
+               torch.as_tensor(p).to(dtype=torch.float32, **dev).contiguous(),
+               torch.as_tensor(v).to(dtype=torch.float32, **dev).contiguous(),
+               torch.as_tensor(w).to(dtype=torch.float32, **dev).contiguous())
+        m = out[3].numel()
+        if out[3].dim() != 1 or m < n or any(tuple(t.shape) != (3, m) for t in out[:3]):
+            raise ValueError(f"launch states must be add, p, v of shape (3, N) and w of shape (N,), N >= {n}")
+        return out
+
     def walk(self, src: Source, n_packets: int, seed: int, tallies: TallySet, *, stream_id=0,
              first_packet=0, wth=1e-4, roulette_m=10.0, fp64=False, normal_fresnel=False,
              block_size=0, grid_blocks=0, packets_per_lane=0, window=(0, 0, 0), quorum=0, hip_stream=None):
@@ -174,6 +188,12 @@ class Walker:
         for i in range(3):
             s.dir[i] = float(src.dir[i])
         s.w0 = float(src.w0)
+        keep = ()
+        if src.states is not None:
+            keep = self._states(src.states, int(n_packets))
+            s.kind = _abi.PMO_BEAM_STATES
+            s.add, s.p, s.v, s.w = (t.data_ptr() for t in keep)
+            s.states_stride = int(keep[3].numel())
         r = _abi.PmoRun()
         r.n_packets = int(n_packets)
         r.first_packet = int(first_packet)
@@ -195,3 +215,4 @@ class Walker:
             rc = self.lib.pmo_walk(C.byref(vol), C.byref(s), C.byref(r), C.byref(t),
                                    _ptr(self.workspace), C.c_void_p(hip_stream))
         _abi.check(rc)
+        self._keep_states = keep          # alive until the asynchronous walk has read them

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), n
         assert n in _abi.SIGNATURES, n
-    assert L.pmo_abi_version() == _abi.ABI_VERSION == 3
+    assert L.pmo_abi_version() == _abi.ABI_VERSION == 4
     assert L.pmo_workspace_bytes() >= 8
 
 
@@ -39,7 +39,8 @@ PROBE = r"""
 int main(void) {
   S(pmo_volume) S(pmo_source) S(pmo_run) S(pmo_tallies) S(pmo_plate_instance) S(pmo_batch_tallies) S(pmo_diff) S(pmo_obd)
   O(pmo_diff, mismatch) O(pmo_plate_instance, n_packets) O(pmo_plate_instance, layers) O(pmo_run, key_dev) O(pmo_run, quorum)
-  O(pmo_volume, optics) O(pmo_source, w0) O(pmo_run, packets_per_lane) O(pmo_tallies, exit_state)
+  O(pmo_volume, optics) O(pmo_source, w0) O(pmo_source, add) O(pmo_source, states_stride)
+  O(pmo_run, packets_per_lane) O(pmo_tallies, exit_state)
   O(pmo_tallies, ring_w) O(pmo_tallies, line_z) O(pmo_tallies, fluence_rz) O(pmo_run, flags) O(pmo_run, window)
   O(pmo_obd, stream) O(pmo_obd, side)
   printf("PMO_S_COUNT %d\nPMO_EV_COUNT %d\nPMO_OBD_STEP_FIELDS %d\n", PMO_S_COUNT, PMO_EV_COUNT, PMO_OBD_STEP_FIELDS);
@@ -62,6 +63,8 @@ def test_ctypes_layout_matches_header(tmp_path):
         assert int(out[name]) == C.sizeof(cls), name
     assert int(out["pmo_volume.optics"]) == _abi.PmoVolume.optics.offset
     assert int(out["pmo_source.w0"]) == _abi.PmoSource.w0.offset
+    assert int(out["pmo_source.add"]) == _abi.PmoSource.add.offset
+    assert int(out["pmo_source.states_stride"]) == _abi.PmoSource.states_stride.offset
     assert int(out["pmo_run.packets_per_lane"]) == _abi.PmoRun.packets_per_lane.offset
     assert int(out["pmo_run.flags"]) == _abi.PmoRun.flags.offset
     assert int(out["pmo_run.window"]) == _abi.PmoRun.window.offset
@@ -92,7 +95,8 @@ def _valid_structs():
 
 @pytest.mark.parametrize("field,value,msg", [
     ("labels", None, "labels"), ("n_labels", 0, "n_labels"), ("pitch", 0.0, "pitch"),
-    ("ix", 0, "launch voxel"), ("kind", 9, "beam kind"), ("block_size", 100, "block_size"),
+    ("ix", 0, "launch voxel"), ("kind", 9, "beam kind"), ("kind", 3, "per-packet launch states"),
+    ("block_size", 100, "block_size"),
     ("roulette_m", 0.5, "roulette_m"), ("scalars", None, "scalars"), ("nx", 2, "extents"),
 ])
 def test_argument_errors_are_codes(field, value, msg):

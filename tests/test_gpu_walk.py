@@ -320,10 +320,14 @@ def test_compat_vmc_kernel_is_a_drop_in(slab):
     r = np.sqrt(x[rd] ** 2 + y[rd] ** 2)
     h, _ = np.histogram(r, bins=np.arange(51) * 0.1, weights=w2[rd].astype(np.float64))
     assert np.allclose(h, ref["rd_r"], rtol=1e-5, atol=1e-6)
-    with pytest.raises(ValueError):
-        p3 = p.copy()
-        p3[0, 5] = 0.01
-        vmc_kernel(add, p3, v, w, ma, ms, n, g, vm, wl.pitch, N, 1)
+    # launch arrays that differ per packet are walked as given (PMO_BEAM_STATES): a packet's
+    # history depends only on its id and its own launch, so only packet 5 changes
+    p3 = p.copy()
+    p3[0, 5] = 0.01
+    a3, p3o, v3, w3 = vmc_kernel(add, p3, v, w, ma, ms, n, g, vm, wl.pitch, N, 1, rand_seed=21)
+    same = np.arange(N) != 5
+    assert np.array_equal(w3[same], w2[same]) and np.array_equal(v3[:, same], v2[:, same])
+    assert np.array_equal(p3o[:, same], p2[:, same])
 
 
 def test_flat_beam_launches_on_voxel_faces_finish(torch_cuda):
@@ -345,3 +349,70 @@ def test_flat_beam_launches_on_voxel_faces_finish(torch_cuda):
         assert ev["escaped"] + ev["killed"] == N
         assert (ev["interactions"] + ev["crossings"]) / N < 5000
         assert ledger_error(r["scalars"]) < 1e-6
+
+
+def _pencil_states(wl, N):
+    ctr = wl.source.voxel
+    add = np.tile(np.array(ctr, np.int32)[:, None], (1, N))
+    p = np.tile(np.array([0.0, 0.0, -wl.pitch / 2], np.float32)[:, None], (1, N))
+    v = np.tile(np.array(wl.source.dir, np.float32)[:, None], (1, N))
+    return add, p, v, np.full(N, wl.source.w0, np.float32)
+
+
+def _tem00_states(wl, N, waist, seed):
+    """A TEM00 beam drawn on the host as the reference draws it: x, y ~ N(0, w/2) around the
+    top-face centre (pymopt/voxel/_classes.py:226-233), turned into voxel address and
+    in-voxel position."""
+    add, p, v, w = _pencil_states(wl, N)
+    l = wl.pitch
+    xy = np.random.default_rng(seed).normal(0.0, waist / 2, (2, N))
+    d = np.floor(xy / l + 0.5)
+    add[:2] += d.astype(np.int32)
+    p[:2] = (xy - l * d).astype(np.float32)
+    return add, p, v, w
+
+
+def test_per_packet_launch_states(slab, oracle):
+    """PMO_BEAM_STATES walks the caller's (add, p, v, w) arrays: arrays that all hold the
+    pencil launch give the pencil walk, and a host-drawn TEM00 beam matches the oracle
+    walking the same arrays."""
+    import dataclasses
+    wl, wk = slab
+    N = 100000
+    a = gpu_run(wk, wl, N, 21)
+    st = dataclasses.replace(wl, source=dataclasses.replace(wl.source, states=_pencil_states(wl, N)))
+    b = gpu_run(wk, st, N, 21)
+    assert a["events"] == b["events"]
+    for k in ("Rd", "Tt", "A", "w0"):
+        assert b["scalars"][k] == pytest.approx(a["scalars"][k], rel=1e-12), k
+    assert np.allclose(a["absorbed"], b["absorbed"], rtol=1e-10, atol=1e-15)
+    N = 20000
+    G, O = [], []
+    for s in range(10):
+        tem = dataclasses.replace(wl, source=dataclasses.replace(wl.source, states=_tem00_states(wl, N, 0.5, s)))
+        G.append(gpu_run(wk, tem, N, 40 + s))
+        O.append(oracle_run(oracle, tem, N, 40 + s, absorbed=False))
+    for k in ("Rd", "Tt", "A"):
+        g = [r["scalars"][k] / N for r in G]
+        o = [r["scalars"][k] / N for r in O]
+        assert zscore(g, o) < 3, k
+        assert np.max(np.abs(np.subtract(g, o))) < np.std(o, ddof=1), k     # same draws
+    zr = binned_z([r["rd_r"] for r in G], [r["rd_r"] for r in O])
+    assert np.sum(zr > 3) <= 2 and np.all(zr < 5)
+
+
+def test_compat_takes_per_packet_launch_arrays(slab):
+    """compat.vmc_kernel with the reference's Gaussian-beam arrays: the exits it returns
+    carry the tallies of the device walk of the same arrays and seed."""
+    import dataclasses
+    from pymonteopt_amd.compat import vmc_kernel
+    wl, wk = slab
+    N = 50000
+    add, p, v, w = _tem00_states(wl, N, 0.5, 3)
+    ma, ms, g, n = wl.optics_columns()
+    add_o, p_o, v_o, w_o = vmc_kernel(add, p, v, w, ma, ms, g=g, n=n, voxel_model=wl.labels, l=wl.pitch,
+                                      nPh=N, end_point=wl.n_labels, rand_seed=8)
+    tem = dataclasses.replace(wl, source=dataclasses.replace(wl.source, states=(add, p, v, w)))
+    r = gpu_run(wk, tem, N, 8)
+    assert w_o[v_o[2] < 0].sum(dtype=np.float64) == pytest.approx(r["scalars"]["Rd"], rel=1e-6)
+    assert w_o[v_o[2] > 0].sum(dtype=np.float64) == pytest.approx(r["scalars"]["Tt"], rel=1e-6)

@@ -125,3 +125,47 @@ def test_exit_tallies_match_their_definitions(oracle):
         near = top & ((np.abs(r - a) < 1e-5) | (np.abs(r - b) < 1e-5))
         assert abs(out["ring_w"][i] - w[sel].sum()) <= w[near].sum() + rel * w[sel].sum() + 1e-12
         assert out["ring_w"][i] > 0
+
+
+def _states(ctr, l, N, w0=0.97):
+    add = np.tile(np.array(ctr, np.int32)[:, None], (1, N))
+    p = np.tile(np.array([0.0, 0.0, -l / 2], np.float32)[:, None], (1, N))
+    v = np.tile(np.array([0.0, 0.0, 1.0], np.float32)[:, None], (1, N))
+    return add, p, v, np.full(N, w0, np.float32)
+
+
+def test_per_packet_states_equal_to_the_pencil_give_the_pencil_walk(oracle):
+    """kind 3 reads the reference kernel's launch arrays; arrays that all hold the pencil
+    launch give the pencil walk bit for bit (same draws, same arithmetic)."""
+    vol, l, ma, ms, g, n, ctr = _slab()
+    N = 3000
+    pencil = dict(kind=0, voxel=ctr, w0=float(np.float32(0.97)))
+    states = dict(kind=0, voxel=ctr, w0=0.0, states=_states(ctr, l, N))
+    # one thread, so that the sums are taken in the same order
+    a = oracle.walk(vol, l, ma, ms, g, n, pencil, N, 9, nr=20, dr=0.1, threads=1)
+    b = oracle.walk(vol, l, ma, ms, g, n, states, N, 9, nr=20, dr=0.1, threads=1)
+    assert a["scalars"] == b["scalars"]
+    assert a["events"] == b["events"]
+    assert np.array_equal(a["absorbed"], b["absorbed"]) and np.array_equal(a["rd_r"], b["rd_r"])
+
+
+def test_per_packet_states_weights_positions_and_end_voxels(oracle):
+    """Each packet starts with its own weight; one placed in the end-label shell leaves at
+    once along its v_z, with no interaction; positions outside the voxel are clamped."""
+    vol, l, ma, ms, g, n, ctr = _slab()
+    N = 4
+    add, p, v, w = _states(ctr, l, N)
+    w[:] = [0.5, 0.25, 0.125, 1.0]
+    add[:, 3] = (0, ctr[1], 1)               # x = 0 is the end shell
+    v[:, 3] = (0.0, 0.6, -0.8)               # leaves upward: counts as Rd
+    p[0, 1] = 5.0                            # clamped to l / 2
+    r = oracle.walk(vol, l, ma, ms, g, n, dict(kind=0, voxel=ctr, w0=0.0, states=(add, p, v, w)), N, 1,
+                    exit_state=True)
+    s = r["scalars"]
+    assert s["w0"] == pytest.approx(1.875, abs=1e-12)
+    assert abs(s["w0"] - (s["Rd"] + s["Tt"] + s["other"] + s["A"] + s["kill"] - s["gain"])) < 1e-12
+    es = r["exit_state"]
+    assert es[6, 3] == pytest.approx(1.0) and es[5, 3] == pytest.approx(-0.8)
+    assert s["Rd"] >= 1.0
+    # the packet in the end voxel is escaped, not walked: 4 launched, all accounted for
+    assert r["events"]["packets"] == 4 and r["events"]["escaped"] + r["events"]["killed"] == 4
