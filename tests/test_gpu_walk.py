@@ -416,3 +416,26 @@ def test_compat_takes_per_packet_launch_arrays(slab):
     r = gpu_run(wk, tem, N, 8)
     assert w_o[v_o[2] < 0].sum(dtype=np.float64) == pytest.approx(r["scalars"]["Rd"], rel=1e-6)
     assert w_o[v_o[2] > 0].sum(dtype=np.float64) == pytest.approx(r["scalars"]["Tt"], rel=1e-6)
+
+
+def test_packet_range_shards_sum_to_the_whole_run(slab):
+    """The multi-GPU design gives each GPU a range of packet ids under one Philox key. On one
+    GPU: four launches over [k N/4, (k+1) N/4) (first_packet) into one set of tallies give
+    the single launch's tallies, since every packet's history depends on its id alone."""
+    import torch
+    from pymonteopt_amd.runtime import TallySpec
+    wl, wk = slab
+    N = 300000
+    spec = TallySpec(**wl.tallies)
+    whole = gpu_run(wk, wl, N, 7)
+    tal = wk.tallies(spec)
+    for k in range(4):
+        a, b = k * N // 4, (k + 1) * N // 4
+        wk.walk(wl.source, b - a, 7, tal, first_packet=a)
+    torch.cuda.synchronize()
+    parts = tal.to_host(shape=wk.shape)
+    assert parts["events"] == whole["events"]
+    for k in ("Rd", "Tt", "A", "w0"):
+        assert parts["scalars"][k] == pytest.approx(whole["scalars"][k], rel=1e-12), k
+    assert np.allclose(parts["absorbed"], whole["absorbed"], rtol=1e-10, atol=1e-15)
+    assert np.allclose(parts["rd_r"], whole["rd_r"], rtol=1e-10, atol=0)
