@@ -36,6 +36,7 @@ struct DiffParams {
     uint32_t seed, stream;
     float wth, roulette_m, inv_m;
     uint32_t flags, batch, max_events;
+    uint32_t quorum;               // lanes ready for an interaction before a wave runs that step
     int32_t mode;                  // 0 forward, 1 replay
     double* scalars;               // [kBS_COUNT] (forward)
     unsigned long long* events;    // [kBE_COUNT] or null
@@ -204,24 +205,38 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
             }
         }
         if (__ballot(alive) == 0ull) break;
+
+        // classify as the plain walk does: each live lane's next event is a face crossing or
+        // the end of its hop; the wave runs one kind per step (interactions once `quorum`
+        // lanes want one or none crosses). A lane not served waits unchanged, so every
+        // packet's events and draws are those of any other order, and the replay retraces
+        // the forward exactly.
+        R db = (R)1000, smt = (R)0;
+        int k = 0;
+        bool want_int = false;
+        const bool walking = alive && lab != end;
+        if (walking) {
+            // only components with |v_i| > 0 take part (_kernel.py:65), as in the reference
+            const R d0 = vx != (R)0 ? (half - (vx > (R)0 ? px : -px)) * ivx : (R)INFINITY;
+            const R d1 = vy != (R)0 ? (half - (vy > (R)0 ? py : -py)) * ivy : (R)INFINITY;
+            const R d2 = vz != (R)0 ? (half - (vz > (R)0 ? pz : -pz)) * ivz : (R)INFINITY;
+            if (d0 < db) { db = d0; k = 0; }
+            if (d1 < db) { db = d1; k = 1; }
+            if (d2 < db) { db = d2; k = 2; }
+            smt = db * mt;
+            want_int = !(st >= smt);
+        }
+        const unsigned long long m_int = __ballot(walking && want_int);
+        const unsigned long long m_cross = __ballot(walking && !want_int);
+        const bool run_int = m_cross == 0ull || (uint32_t)__popcll(m_int) >= P.quorum;
         if (!alive) continue;
 
         bool ended = false, killed = false;
         if (lab == end) {
             ended = true;
-        } else {
+        } else if (want_int == run_int) {
             ++nev;
-            // only components with |v_i| > 0 take part (_kernel.py:65), as in the reference
-            const R d0 = vx != (R)0 ? (half - (vx > (R)0 ? px : -px)) * ivx : (R)INFINITY;
-            const R d1 = vy != (R)0 ? (half - (vy > (R)0 ? py : -py)) * ivy : (R)INFINITY;
-            const R d2 = vz != (R)0 ? (half - (vz > (R)0 ? pz : -pz)) * ivz : (R)INFINITY;
-            R db = (R)1000;
-            int k = 0;
-            if (d0 < db) { db = d0; k = 0; }
-            if (d1 < db) { db = d1; k = 1; }
-            if (d2 < db) { db = d2; k = 2; }
-            const R smt = db * mt;
-            if (st >= smt) {
+            if (!want_int) {
                 ++n_cross;
                 px += vx * db; py += vy * db; pz += vz * db;
                 if (replay) grad_segment(db);

@@ -437,6 +437,8 @@ int diff_launch(const pmo_volume* vol, const pmo_source* src, const pmo_run* run
     P.wth = run->wth; P.roulette_m = run->roulette_m; P.inv_m = 1.f / run->roulette_m;
     P.flags = run->flags;
     P.max_events = kMaxEvents;
+    if (run->quorum < 0 || run->quorum > 64) return fail(PMO_ERR_INVALID, "quorum %d outside [0, 64]", run->quorum);
+    P.quorum = run->quorum > 0 ? (uint32_t)run->quorum : (uint32_t)kDefaultQuorum;
     P.mode = mode;
     P.scalars = scalars;
     P.events = reinterpret_cast<unsigned long long*>(events);
