@@ -240,6 +240,87 @@ def bone_leg(peaks, n_packets=100_000_000):
             "global_deposits_per_packet": glob_pp, "ops_per_packet": ops}
 
 
+# The reference's own published benchmark: a homogeneous 5 mm slab, 5 x 5 mm wide (mu_s 10,
+# mu_a 0.01 /mm, g 0.9, n 1.5, ambient 1.0), pencil beam, timed around start()
+# (test_voxelcuda.ipynb:1409-1423 parameters; pymopt/voxel_gpu/_classes.py:88-127 timed region).
+PUBLISHED_SLAB = dict(thickness=[5], ms=[10], ma=[0.01], g=[0.9], n=[1.5], n_air=1.0, x_size=5, y_size=5)
+# 1e8 packets in 21.68, 21.33 and 22.06 s on an unstated NVIDIA GPU (test_voxelcuda.ipynb:1270-1273,
+# :1403-1406 and cell 37's outputs); Rd, Td printed by the two 1e8 runs
+PUBLISHED_GPU_S_1E8 = (21.680, 21.331, 22.06)
+PUBLISHED_RDTD_1E8 = ((0.527456, 0.344113), (0.528290, 0.343399))
+# pitch sweep: 5e6 packets per pitch, mean of 3 repeats (test_voxelcuda.ipynb:999-1001, cell 82)
+PUBLISHED_PITCH_S = {5: 1.0833, 1: 0.969, 0.5: 1.0657, 0.1: 1.6243, 0.05: 2.182, 0.01: 6.938, 0.005: 12.2017}
+# the Numba CPU kernel, serial, one core: 1e8 packets in 949.6 s (test_voxelcuda.ipynb:1585,1608)
+PUBLISHED_CPU_PER_CORE = 1e8 / 949.6
+
+
+def published_slab_leg(n_packets=100_000_000, sweep_packets=5_000_000):
+    """The reference's published benchmark workloads through the model API, timed as the
+    reference times them: wall time of start() on a freshly built model, which uploads the
+    volume, walks, copies the tallies back and normalises them. The GPU-event time of the
+    walk alone is reported beside it. The sweep repeats the reference's voxel-pitch series
+    (5 mm down to 0.005 mm, a 1002^3 volume) at its packet count, without the absorbed map
+    (the reference's sweep computed only Rd(r), 200 bins of 0.01 mm)."""
+    import torch
+    from pymonteopt_amd.voxel import VoxelPlateModel
+
+    def run(nph, pitch, **kw):
+        m = VoxelPlateModel(nPh=nph, **kw).build(**PUBLISHED_SLAB, voxel_space=pitch)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m.start(0)
+        wall = time.perf_counter() - t0
+        r = m.getRdTtValues()
+        return m, wall, float(r["Rd"]), float(r["Td"])
+
+    run(100_000, 0.5)                                       # code-object load, first allocations
+    m, wall, rd, td = run(n_packets, 0.5, nr=100, dr=0.01)
+    ref_s = statistics.mean(PUBLISHED_GPU_S_1E8)
+    out = {"workload": "5 mm slab, 5 x 5 mm wide @0.5 mm (12^3 padded), pencil, "
+                       f"{n_packets:.0e} packets, absorbed map + Rd(r) 100 x 0.01 mm",
+           "start_wall_s": wall, "photons_per_s": n_packets / wall,
+           "walk_s": m.time_s, "walk_photons_per_s": n_packets / m.time_s,
+           "Rd": rd, "Td": td, "reference_Rd_Td_1e8": [list(x) for x in PUBLISHED_RDTD_1E8],
+           "reference_gpu_photons_per_s": n_packets / ref_s,
+           "speedup_vs_reference_gpu": ref_s / wall,
+           "reference_hardware": "unstated NVIDIA GPU"}
+    sweep = []
+    for pitch, ref in PUBLISHED_PITCH_S.items():
+        m, wall, rd, td = run(sweep_packets, pitch, fluence=False, nr=200, dr=0.01)
+        shape = m.model.voxel_model.shape
+        sweep.append({"pitch_mm": pitch, "volume": list(shape), "start_wall_s": wall,
+                      "walk_s": m.time_s, "photons_per_s": sweep_packets / wall,
+                      "reference_photons_per_s": sweep_packets / ref, "speedup": ref / wall,
+                      "Rd": rd, "Td": td})
+        del m
+        torch.cuda.empty_cache()
+    out["pitch_sweep"] = {"packets": sweep_packets, "points": sweep}
+    return out
+
+
+def cpu_anchor_published_slab(budget_s=8.0):
+    """The oracle on one host thread, float32 state, on the published slab at 0.5 mm, beside
+    the reference's published serial Numba rate (1.053e5 photons/s, unstated CPU): a sanity
+    anchor for the CPU baseline, not a comparison of machines."""
+    from oracle import oracle as og
+    from pymonteopt_amd import presets
+    og.build()
+    wl = presets.plate("published slab", **PUBLISHED_SLAB, voxel_space=0.5)
+    ma, ms, g, n = wl.optics_columns()
+    n_s = 20_000
+    while True:
+        t0 = time.perf_counter()
+        og.walk(wl.labels, wl.pitch, ma, ms, g, n, wl.source.as_dict(), n_s, 31, precision="f32",
+                threads=1, nr=100, dr=0.01)
+        dt = time.perf_counter() - t0
+        if dt >= 0.6 * budget_s or n_s >= 20_000_000:
+            break
+        n_s = int(min(20_000_000, n_s * min(64.0, budget_s / dt)))
+    return {"value": n_s / dt, "unit": "photons/s", "cores": 1, "kind": "port",
+            "sample": f"{n_s} packets of the 5 mm slab @0.5 mm (f32 oracle, absorbed + Rd(r)), {dt:.1f} s",
+            "reference_numba_per_core": PUBLISHED_CPU_PER_CORE}
+
+
 def virtual_obd_leg(n_packets=10_000_000, seeds=(4, 5)):
     """Subjects of the virtual OBD pipeline at the script's sizes (pymopt/virtualOBD.py): a
     300^3 Turing volume grown 100 steps and tiled 2 x 4 x 2, wrapped in the subject's
@@ -515,12 +596,14 @@ def main():
             out["turing"] = turing_leg()
             out["virtual_obd"] = virtual_obd_leg()
             out["bone"] = bone_leg(peaks)
+            out["published_slab"] = published_slab_leg()
         if not args.no_cpu:
             out["cpu_baseline"] = cpu_baseline(wl)
             if not args.no_extra:
                 out["obd"]["cpu_baseline"] = cpu_baseline_obd(*obd_sample)
                 out["obd"]["gpu_vs_cpu"] = out["obd"]["traces_per_s"] / out["obd"]["cpu_baseline"]["value"]
                 out["turing"]["cpu_baseline"] = cpu_baseline_turing()
+                out["published_slab"]["cpu_anchor"] = cpu_anchor_published_slab()
     print(json.dumps(out), flush=True)
 
 

@@ -153,6 +153,29 @@ def test_known_answers_on_gpu(torch_cuda):
     assert v["Td"] == pytest.approx(c["reference_gpu"]["Tt"], abs=0.0012)
 
 
+def test_fine_pitch_slab_matches_published_run(torch_cuda):
+    """The reference's voxel-pitch benchmark at its finest pitch: the 5 mm slab at 0.005 mm
+    (1002^3 labels, 1 GB) with 1e7 packets, against the Rd, Td its GPU run printed at that
+    pitch. 10 seeds of 1e6 packets give the SE; the reference's single 1e7 run is taken to
+    have the same SE."""
+    from pymonteopt_amd.voxel import VoxelPlateModel
+    ka = json.load(open(os.path.join(HERE, "golden", "reference_known_answers.json")))
+    c = ka["voxel_size_invariance"]
+    ref = c["reference_gpu"]["0.005mm"]
+    m = VoxelPlateModel(nPh=1_000_000, nr=0, fluence=False).build(**c["params"], voxel_space=0.005)
+    assert m.model.voxel_model.shape == (1002, 1002, 1002)
+    rd, tt = [], []
+    for s in range(10):
+        m.start(s)
+        v = m.getRdTtValues()
+        rd.append(v["Rd"])
+        tt.append(v["Td"])
+    se = lambda x: np.std(x, ddof=1) / math.sqrt(len(x))
+    for mine, k in ((rd, "Rd"), (tt, "Tt")):
+        pooled = math.sqrt(2.0) * se(mine)
+        assert abs(np.mean(mine) - ref[k]) < 4 * pooled, (k, np.mean(mine), ref[k], pooled)
+
+
 def test_skin_flat_beam_parity(torch_cuda, oracle):
     """3-layer skin, 1 mm flat beam, 2e4 packets x 10 seeds against the oracle."""
     from pymonteopt_amd import presets
