@@ -133,14 +133,16 @@ typedef struct pmo_run {
     float roulette_m;       /* roulette survival 1/m, weight multiplier m (reference 10)       */
     uint32_t flags;         /* PMO_F_*                                                         */
     int32_t block_size;     /* threads per block, multiple of 64, <= 1024; 0 = default (1024 for
-                               runs of >= 2^20 packets with an absorbed map, else 256)          */
+                               runs of >= 2^20 packets with an absorbed map over more than
+                               16*16*8 voxels, else 256)                                       */
     int32_t grid_blocks;    /* persistent blocks; 0 = fill the device at full occupancy        */
     int32_t packets_per_lane; /* a wave claims 64*this packet ids per work-counter atomic; 0 = default */
     int32_t window[3];      /* extents (x, y, z) of the box of the absorbed tally that each block
                                accumulates in LDS, centred laterally on the launch voxel and
-                               starting at its z; {-1,-1,-1} = none; {0,0,0} = default: 28x28x24
+                               starting at its z; {-1,-1,-1} = none; {0,0,0} = default: the
+                               whole volume when it has at most 16*16*8 voxels, else 28x28x24
                                with 1024-lane blocks (the default block size for runs of >= 2^20
-                               packets with an absorbed map), else 16x16x8, shrunk to the LDS the
+                               packets with an absorbed map), else 16x16x8; shrunk to the LDS the
                                histograms leave                                                 */
     const uint32_t* key_dev; /* device (seed, stream) read by the kernel instead of seed/stream when
                                 non-NULL, so a captured HIP graph can be replayed with new keys */

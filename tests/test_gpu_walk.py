@@ -300,6 +300,27 @@ def test_wide_launch_is_invisible(slab):
     assert r["fluence_rz"].sum() > 0
 
 
+def test_whole_volume_window_is_invisible(torch_cuda):
+    """A volume of at most 16*16*8 voxels (the 5 mm slab at 0.5 mm, 12^3 padded) stays on
+    256-lane blocks at 2^20 packets and more, with the whole volume in the LDS window: no
+    global deposits, and the same tallies as a run without a window."""
+    from pymonteopt_amd import presets
+    from pymonteopt_amd.runtime import Walker
+    wl = presets.plate("slab 12^3", [5], [0.01], [10], [0.9], [1.5], 1.0, 0.5, 5, 5,
+                       tallies=dict(nr=100, dr=0.01))
+    wk = Walker(wl.labels, wl.pitch, wl.optics)
+    N = (1 << 20) + 77
+    base = gpu_run(wk, wl, N, 5, block_size=256, window=(-1, -1, -1))
+    r = gpu_run(wk, wl, N, 5)
+    assert r["events"]["global_deposits"] == 0
+    assert base["events"]["global_deposits"] > 0
+    path_events = lambda e: {k: v for k, v in e.items() if k != "global_deposits"}
+    assert path_events(r["events"]) == path_events(base["events"])
+    assert np.allclose(r["absorbed"], base["absorbed"], rtol=1e-10, atol=1e-15)
+    for k in ("Rd", "Tt", "A"):
+        assert r["scalars"][k] == pytest.approx(base["scalars"][k], rel=1e-12)
+
+
 def test_event_scheduling_is_invisible(slab):
     """Which step a wave runs first (quorum) changes only the interleaving of lanes, not
     any packet's own event sequence: tallies agree to FP64 summation order."""
