@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define PMO_ABI_VERSION 4
+#define PMO_ABI_VERSION 5
 
 /* ---- error codes ---------------------------------------------------------------- */
 #define PMO_OK 0
@@ -104,6 +104,11 @@ typedef struct pmo_volume {
     float pitch;           /* voxel edge (mm), the reference's voxel_space / l                 */
     const float* optics;   /* device, float4 per label, n_labels+1 entries: (mu_a, mu_s, g, n);
                               entry n_labels holds n_ambient (mu_a, mu_s, g unused)            */
+    const uint8_t* skip;   /* device [nx][ny][nz], pmo_skip_radius of these labels, or NULL.
+                              With it pmo_walk crosses the faces inside same-label regions in
+                              one step (the same draws and events; merged arithmetic); NULL
+                              steps voxel by voxel as the reference does. Other entry points
+                              ignore it.                                                       */
 } pmo_volume;
 
 typedef struct pmo_source {
@@ -243,6 +248,18 @@ int pmo_device_info(int32_t* n_cu, int32_t* blocks_per_cu_walk, int32_t block_si
  * detectors (pymopt/virtualOBD.py:158-172). */
 int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run,
              const pmo_tallies* tal, void* workspace, void* stream);
+
+/* Largest radius pmo_skip_radius accepts */
+#define PMO_SKIP_MAX_CAP 127
+
+/* radius[v] = min(cap, Chebyshev distance from voxel v to the nearest voxel that lies on the
+ * volume's outer face or has a 26-neighbour of another label): the cube of half-width
+ * radius[v] voxels around v holds v's label only. labels is the walk's volume; radius and
+ * scratch are device buffers of nx*ny*nz bytes owned by the caller (scratch is overwritten).
+ * 1 <= cap <= PMO_SKIP_MAX_CAP. Feeds pmo_volume.skip; no reference counterpart: the
+ * reference steps every voxel face (pymopt/voxel/_kernel.py:62-125). */
+int pmo_skip_radius(const uint8_t* labels, int32_t nx, int32_t ny, int32_t nz, int32_t cap, uint8_t* radius,
+                    uint8_t* scratch, void* stream);
 
 /* Diagnostic: out[4*i .. 4*i+3] = Philox4x32-10(ctr[4*i .. 4*i+3], key) on the device, the
  * generator every walk draw comes from (no reference counterpart: the reference draws from

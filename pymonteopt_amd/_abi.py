@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpmo.so")
 
-ABI_VERSION = 4          # PMO_ABI_VERSION of include/pmo.h
+ABI_VERSION = 5          # PMO_ABI_VERSION of include/pmo.h
 
 PMO_OK = 0
 PMO_ERR_INVALID = -1
@@ -28,6 +28,8 @@ PMO_F_FP64 = 1 << 0
 PMO_F_NORMAL_FRESNEL = 1 << 1
 PMO_F_LAYER_STEP = 1 << 2
 
+PMO_SKIP_MAX_CAP = 127
+
 PMO_PROBE_F64, PMO_PROBE_F32, PMO_PROBE_U64, PMO_PROBE_U32, PMO_PROBE_F64_PER_XCD = 0, 1, 2, 3, 4
 
 SCALAR_NAMES = ["Rd", "Tt", "other", "A", "kill", "gain", "w0",
@@ -40,7 +42,7 @@ PMO_EV_COUNT = 8
 
 class PmoVolume(C.Structure):
     _fields_ = [("labels", C.c_void_p), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
-                ("n_labels", C.c_int32), ("pitch", C.c_float), ("optics", C.c_void_p)]
+                ("n_labels", C.c_int32), ("pitch", C.c_float), ("optics", C.c_void_p), ("skip", C.c_void_p)]
 
 
 class PmoSource(C.Structure):
@@ -114,6 +116,8 @@ SIGNATURES = {
     "pmo_walk": ([C.POINTER(PmoVolume), C.POINTER(PmoSource), C.POINTER(PmoRun), C.POINTER(PmoTallies),
                   C.c_void_p, C.c_void_p], C.c_int),
     "pmo_philox": ([C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
+    "pmo_skip_radius": ([C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                         C.c_void_p], C.c_int),
     "pmo_walk_forward_diff": ([C.POINTER(PmoVolume), C.POINTER(PmoSource), C.POINTER(PmoRun), C.c_void_p,
                                C.c_void_p, C.POINTER(PmoDiff), C.c_void_p, C.c_void_p], C.c_int),
     "pmo_walk_adjoint": ([C.POINTER(PmoVolume), C.POINTER(PmoSource), C.POINTER(PmoRun), C.POINTER(PmoDiff),

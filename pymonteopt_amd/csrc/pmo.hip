@@ -15,6 +15,7 @@
 #include "adjoint.hpp"
 #include "batch.hpp"
 #include "obd.hpp"
+#include "skip.hpp"
 #include "turing.hpp"
 #include "walk.hpp"
 
@@ -90,11 +91,16 @@ int device_info(DevInfo* out)
 }
 
 template <typename R>
-int blocks_per_cu(int block, size_t lds, int* out, bool rz = false)
+int blocks_per_cu(int block, size_t lds, int* out, bool rz = false, bool skip = false)
 {
     int nb = 0;
-    hipError_t e = rz ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::walk_kernel<R, true>, block, lds)
-                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::walk_kernel<R, false>, block, lds);
+    hipError_t e;
+    if (skip)
+        e = rz ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::walk_kernel<R, true, true>, block, lds)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::walk_kernel<R, false, true>, block, lds);
+    else
+        e = rz ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::walk_kernel<R, true, false>, block, lds)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::walk_kernel<R, false, false>, block, lds);
     if (e != hipSuccess) return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     *out = nb > 0 ? nb : 1;
     return PMO_OK;
@@ -182,6 +188,7 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     pmo::WalkParams P;
     memset(&P, 0, sizeof(P));
     P.labels = vol->labels;
+    P.skip = vol->skip;
     P.optics = reinterpret_cast<const float4*>(vol->optics);
     P.nx = (uint32_t)vol->nx; P.ny = (uint32_t)vol->ny; P.nz = (uint32_t)vol->nz;
     P.nvox = (uint32_t)nvox;
@@ -292,8 +299,9 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
         return fail(PMO_ERR_UNSUPPORTED, "histograms and tally window need %zu B of LDS (limit %zu)", lds, d.max_lds);
 
     int nb = 0;
-    rc = fp64 ? blocks_per_cu<double>(bs, lds, &nb, P.fluence_rz != nullptr)
-              : blocks_per_cu<float>(bs, lds, &nb, P.fluence_rz != nullptr);
+    const bool skip = P.skip != nullptr;
+    rc = fp64 ? blocks_per_cu<double>(bs, lds, &nb, P.fluence_rz != nullptr, skip)
+              : blocks_per_cu<float>(bs, lds, &nb, P.fluence_rz != nullptr, skip);
     if (rc != PMO_OK) return rc;
     unsigned long long grid = run->grid_blocks > 0 ? (unsigned long long)run->grid_blocks
                                                    : (unsigned long long)nb * (unsigned long long)d.n_cu;
@@ -316,18 +324,53 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     hipLaunchKernelGGL(pmo::zero_u64_kernel, dim3(1), dim3(64), 0, s, P.work, 4);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "zero workspace launch");
-    if (fp64)
-        if (P.fluence_rz)
-            hipLaunchKernelGGL((pmo::walk_kernel<double, true>), dim3((unsigned)grid), dim3(bs), lds, s, P);
-        else
-            hipLaunchKernelGGL((pmo::walk_kernel<double, false>), dim3((unsigned)grid), dim3(bs), lds, s, P);
-    else
-        if (P.fluence_rz)
-            hipLaunchKernelGGL((pmo::walk_kernel<float, true>), dim3((unsigned)grid), dim3(bs), lds, s, P);
-        else
-            hipLaunchKernelGGL((pmo::walk_kernel<float, false>), dim3((unsigned)grid), dim3(bs), lds, s, P);
+    const dim3 gd((unsigned)grid), bd(bs);
+    const bool rz = P.fluence_rz != nullptr;
+    if (fp64) {
+        if (skip) {
+            if (rz) hipLaunchKernelGGL((pmo::walk_kernel<double, true, true>), gd, bd, lds, s, P);
+            else hipLaunchKernelGGL((pmo::walk_kernel<double, false, true>), gd, bd, lds, s, P);
+        } else {
+            if (rz) hipLaunchKernelGGL((pmo::walk_kernel<double, true, false>), gd, bd, lds, s, P);
+            else hipLaunchKernelGGL((pmo::walk_kernel<double, false, false>), gd, bd, lds, s, P);
+        }
+    } else {
+        if (skip) {
+            if (rz) hipLaunchKernelGGL((pmo::walk_kernel<float, true, true>), gd, bd, lds, s, P);
+            else hipLaunchKernelGGL((pmo::walk_kernel<float, false, true>), gd, bd, lds, s, P);
+        } else {
+            if (rz) hipLaunchKernelGGL((pmo::walk_kernel<float, true, false>), gd, bd, lds, s, P);
+            else hipLaunchKernelGGL((pmo::walk_kernel<float, false, false>), gd, bd, lds, s, P);
+        }
+    }
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "walk launch");
+    return PMO_OK;
+}
+
+int pmo_skip_radius(const uint8_t* labels, int32_t nx, int32_t ny, int32_t nz, int32_t cap, uint8_t* radius,
+                    uint8_t* scratch, void* stream)
+{
+    g_err[0] = '\0';
+    if (!labels || !radius || !scratch) return fail(PMO_ERR_INVALID, "labels, radius and scratch must be non-null");
+    if (nx < 1 || ny < 1 || nz < 1) return fail(PMO_ERR_INVALID, "volume extents must be >= 1");
+    const unsigned long long n = (unsigned long long)nx * ny * nz;
+    if (n >= (1ull << 31)) return fail(PMO_ERR_UNSUPPORTED, "volume of %llu voxels exceeds 2^31", n);
+    if (cap < 1 || cap > PMO_SKIP_MAX_CAP) return fail(PMO_ERR_INVALID, "cap %d outside [1, %d]", cap, PMO_SKIP_MAX_CAP);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const uint32_t un = (uint32_t)n, c = (uint32_t)cap;
+    const unsigned grid = (unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
+    // b -> scratch; min-max along z -> radius; along y -> scratch; along x -> radius (clamped)
+    hipLaunchKernelGGL(pmo::skip_seed_kernel, dim3(grid), dim3(256), 0, s, labels, (uint32_t)nx, (uint32_t)ny,
+                       (uint32_t)nz, c, scratch);
+    hipLaunchKernelGGL(pmo::skip_pass_kernel, dim3(grid), dim3(256), 0, s, (const uint8_t*)scratch, radius, un,
+                       (uint32_t)nz, 1u, c, 0u);
+    hipLaunchKernelGGL(pmo::skip_pass_kernel, dim3(grid), dim3(256), 0, s, (const uint8_t*)radius, scratch, un,
+                       (uint32_t)ny, (uint32_t)nz, c, 0u);
+    hipLaunchKernelGGL(pmo::skip_pass_kernel, dim3(grid), dim3(256), 0, s, (const uint8_t*)scratch, radius, un,
+                       (uint32_t)nx, (uint32_t)ny * (uint32_t)nz, c, 1u);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "skip radius launch");
     return PMO_OK;
 }
 

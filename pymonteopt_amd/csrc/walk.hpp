@@ -38,6 +38,7 @@ template <> struct Mth<float> {
     static __device__ __forceinline__ float fmax(float a, float b) { return __builtin_fmaxf(a, b); }
     static __device__ __forceinline__ float fmin(float a, float b) { return __builtin_fminf(a, b); }
     static __device__ __forceinline__ float floor(float x) { return __builtin_floorf(x); }
+    static __device__ __forceinline__ float rint(float x) { return __builtin_rintf(x); }
 };
 
 template <> struct Mth<double> {
@@ -55,6 +56,7 @@ template <> struct Mth<double> {
     static __device__ __forceinline__ double fmax(double a, double b) { return __builtin_fmax(a, b); }
     static __device__ __forceinline__ double fmin(double a, double b) { return __builtin_fmin(a, b); }
     static __device__ __forceinline__ double floor(double x) { return __builtin_floor(x); }
+    static __device__ __forceinline__ double rint(double x) { return __builtin_rint(x); }
 };
 
 // per-label constants in LDS, derived once per block from (mu_a, mu_s, g, n)
@@ -76,6 +78,7 @@ template <typename R> struct LabelConst {
 // ------------------------------------------------------------------ kernel parameters
 struct WalkParams {
     const uint8_t* labels;
+    const uint8_t* skip;       // uniform-cube radius per voxel (skip_*_kernel), or null
     const float4* optics;
     uint32_t nx, ny, nz, nvox;
     uint32_t sx, sy;           // strides of x and y in the C-order volume
@@ -298,10 +301,33 @@ __device__ __forceinline__ void launch_xy(int kind, float radius, R pitch, R hal
     if (ay > ny - 2) { ay = ny - 2; py = half; }
 }
 
+// A position given relative to the centre of a voxel inside a uniform cube of half-width
+// rad voxels, moved into the frame of the cube's voxel that holds it: returns the voxel
+// offset, in [-rad, rad], and leaves p in [-half, half].
+template <typename R>
+__device__ __forceinline__ int cube_relocate(R& p, R pitch, R ipitch, R half, int rad)
+{
+    using M = Mth<R>;
+    int o = (int)M::rint(p * ipitch);
+    o = o < -rad ? -rad : (o > rad ? rad : o);
+    p = M::fmin(M::fmax(p - (R)o * pitch, -half), half);
+    return o;
+}
+
+__device__ __forceinline__ int32_t clamp_index(int32_t a, uint32_t n)
+{
+    return a < 0 ? 0 : (a >= (int32_t)n ? (int32_t)n - 1 : a);
+}
+
 // ------------------------------------------------------------------ the walk kernel
 // RZ: also tally the r-z fluence (a separate instantiation, so the plain walk keeps its
-// register budget)
-template <typename R, bool RZ>
+// register budget). SKIP: P.skip holds each voxel's uniform-cube radius, and a packet in a
+// voxel of radius r > 0 treats the cube of half-width r around it as one voxel: one step
+// reaches its face or the hop's end. The faces inside the cube join voxels of one label,
+// where the reference moves on without a draw (_kernel.py:82), so every packet makes the
+// same draws and the same events; only the arithmetic of the merged steps is rounded once
+// instead of per face.
+template <typename R, bool RZ, bool SKIP>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) walk_kernel(const WalkParams P)
 {
     using M = Mth<R>;
@@ -324,6 +350,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
     const R half = (R)P.pitch * (R)0.5;
     const R pitch = (R)P.pitch;
+    const R ipitch = (R)1 / pitch;
     const bool normal_limit = (P.flags & 2u) != 0u;
     const uint32_t seed = P.key_dev ? __builtin_amdgcn_readfirstlane(P.key_dev[0]) : P.seed;
     const uint32_t skey = P.key_dev ? __builtin_amdgcn_readfirstlane(P.key_dev[1]) : P.stream;
@@ -334,13 +361,15 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
     R w = 0, st = 0, mt = 0, ni = 0, u_roul = 0, u_th = 0, u_phi = 0;
     R fb0 = 0, fb1 = 0, fb2 = 0;
     int32_t ax = 0, ay = 0, az = 0;    // voxel address (reference add[0..2])
+    int32_t rad = 0;                   // uniform-cube radius around the voxel (SKIP), a lower bound
     uint32_t lab = 0, inter = 0, fres = 0, nev = 0;
     unsigned long long pid = 0;
     // lane accumulators (kept across packets)
     double acc_abs = 0.0;
     int32_t dep_slot = kNoSlot;      // where the pending deposit goes (see flush_deposit)
     double dep_acc = 0.0;
-    uint32_t n_cross = 0, n_pk = 0, n_glob = 0;
+    uint32_t n_cross = 0, n_glob = 0;
+    uint32_t n_pk = 0;               // packets this wave launched (wave-uniform)
 
     // wave-uniform work batch
     unsigned long long b_next = 0, b_end = 0;
@@ -354,6 +383,9 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
                 unsigned long long claim = 0;
                 if (lane == 0) claim = atomicAdd(P.work, (unsigned long long)P.batch);
                 claim = __shfl(claim, 0);
+                // wave-uniform: kept in scalar registers
+                claim = ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(claim >> 32)) << 32) |
+                        (unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)claim);
                 if (claim >= P.n_packets) {
                     exhausted = true;
                 } else {
@@ -387,8 +419,12 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
                     }
                     ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
                     uint32_t l0 = end;   // an address outside the volume leaves at once
-                    if ((uint32_t)ax < P.nx && (uint32_t)ay < P.ny && (uint32_t)az < P.nz)
-                        l0 = P.labels[((uint32_t)ax * P.ny + (uint32_t)ay) * P.nz + (uint32_t)az];
+                    rad = 0;
+                    if ((uint32_t)ax < P.nx && (uint32_t)ay < P.ny && (uint32_t)az < P.nz) {
+                        const uint32_t li = ((uint32_t)ax * P.ny + (uint32_t)ay) * P.nz + (uint32_t)az;
+                        l0 = P.labels[li];
+                        if (SKIP) rad = P.skip[li];
+                    }
                     lab = l0 < end ? l0 : end;
                     ni = tab[lab].n;
                     mt = tab[lab].mt;
@@ -397,9 +433,11 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
                     st = M::neglog(u01<R>(r.x));
                     u_roul = u01<R>(r.y); u_th = u01<R>(r.z); u_phi = u01<R>(r.w);
                     alive = true;
-                    ++n_pk;
                 }
-                b_next += (uint32_t)__popcll(need) < avail ? (uint32_t)__popcll(need) : avail;
+                const uint32_t launched = (uint32_t)__popcll(need) < avail ? (uint32_t)__popcll(need) : avail;
+                b_next += launched;
+                n_pk += launched;                 // wave-uniform
+
             }
         }
         if (__ballot(alive) == 0ull) break;
@@ -417,10 +455,12 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
         const bool walking = alive && lab != end;
         if (walking) {
             // distance to the exit face, x < y < z on ties (_kernel.py:63-68)
-            // only components with |v_i| > 0 take part (_kernel.py:65), as in the reference
-            const R d0 = vx != (R)0 ? (half - (vx > (R)0 ? px : -px)) * ivx : (R)INFINITY;
-            const R d1 = vy != (R)0 ? (half - (vy > (R)0 ? py : -py)) * ivy : (R)INFINITY;
-            const R d2 = vz != (R)0 ? (half - (vz > (R)0 ? pz : -pz)) * ivz : (R)INFINITY;
+            // only components with |v_i| > 0 take part (_kernel.py:65), as in the reference;
+            // with SKIP, the exit face of the uniform cube around the voxel
+            const R ext = SKIP ? (R)rad * pitch + half : half;
+            const R d0 = vx != (R)0 ? (ext - (vx > (R)0 ? px : -px)) * ivx : (R)INFINITY;
+            const R d1 = vy != (R)0 ? (ext - (vy > (R)0 ? py : -py)) * ivy : (R)INFINITY;
+            const R d2 = vz != (R)0 ? (ext - (vz > (R)0 ? pz : -pz)) * ivz : (R)INFINITY;
             if (d0 < db) { db = d0; k = 0; }
             if (d1 < db) { db = d1; k = 1; }
             if (d2 < db) { db = d2; k = 2; }
@@ -441,7 +481,23 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
             ended = true; xk = 2; xs = vz > (R)0 ? 1 : -1;
         } else if (want_int == run_int) {
             ++nev;
-            if (!want_int) {
+            if (!want_int && SKIP && rad > 0) {
+                // ------------------------------------------------ to the uniform cube's face
+                px += vx * db; py += vy * db; pz += vz * db;
+                const R vk = k == 0 ? vx : (k == 1 ? vy : vz);
+                st -= smt;
+                const int s = vk > (R)0 ? 1 : -1;
+                int ox = cube_relocate<R>(px, pitch, ipitch, half, rad);
+                int oy = cube_relocate<R>(py, pitch, ipitch, half, rad);
+                int oz = cube_relocate<R>(pz, pitch, ipitch, half, rad);
+                const R face = M::copysign(half, vk);
+                if (k == 0) { ox = s * rad; px = face; }
+                else if (k == 1) { oy = s * rad; py = face; }
+                else { oz = s * rad; pz = face; }
+                ax = clamp_index(ax + ox, P.nx); ay = clamp_index(ay + oy, P.ny); az = clamp_index(az + oz, P.nz);
+                n_cross += (uint32_t)(abs(ox) + abs(oy) + abs(oz));
+                rad = 0;            // on the cube's face: the next step crosses it voxel by voxel
+            } else if (!want_int) {
                 // ------------------------------------------------ face crossing (:69-120)
                 ++n_cross;
                 px += vx * db; py += vy * db; pz += vz * db;
@@ -454,9 +510,11 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
                 const int32_t by = ay + (k == 1 ? s : 0);
                 const int32_t bz = az + (k == 2 ? s : 0);
                 uint32_t nxt = end;
+                int32_t nrad = 0;
                 if ((uint32_t)bx < P.nx && (uint32_t)by < P.ny && (uint32_t)bz < P.nz) {
                     const uint32_t nvi = (__umul24((uint32_t)bx, P.ny) + (uint32_t)by) * P.nz + (uint32_t)bz;
                     nxt = P.labels[nvi];
+                    if (SKIP) nrad = P.skip[nvi];
                     nxt = nxt < end ? nxt : end;
                 }
                 const R nt = tab[nxt].n;
@@ -485,6 +543,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
                     ax = bx; ay = by; az = bz;
                     if (k == 0) px = -px; else if (k == 1) py = -py; else pz = -pz;
                     lab = nxt;
+                    if (SKIP) rad = nrad;
                     if (lab == end) {
                         ended = true; xk = k; xs = s;
                     } else {
@@ -498,6 +557,17 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
                 // ------------------------------------------------ hop ends in the voxel
                 const R sm = M::div(st, mt);
                 px += vx * sm; py += vy * sm; pz += vz * sm;       // (:122-124)
+                if (SKIP && rad > 0) {
+                    // the hop ended inside the uniform cube: into the frame of the voxel that
+                    // holds its end; the cube around that voxel is at least rad - offset
+                    const int ox = cube_relocate<R>(px, pitch, ipitch, half, rad);
+                    const int oy = cube_relocate<R>(py, pitch, ipitch, half, rad);
+                    const int oz = cube_relocate<R>(pz, pitch, ipitch, half, rad);
+                    ax = clamp_index(ax + ox, P.nx); ay = clamp_index(ay + oy, P.ny); az = clamp_index(az + oz, P.nz);
+                    const int mx = abs(ox), my = abs(oy), mz = abs(oz);
+                    n_cross += (uint32_t)(mx + my + mz);
+                    rad -= mx > my ? (mx > mz ? mx : mz) : (my > mz ? my : mz);
+                }
                 const LabelConst<R>& L = tab[lab];
                 const R wn = w - w * L.ra;                         // drop (:130)
                 const double dep = (double)w - (double)wn;        // exact in FP64
@@ -623,9 +693,11 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
     // ---------------------------------------------------------------- block flush
     if (P.absorbed) flush_deposit(P, win, dep_slot, dep_acc, n_glob);
     lds_add(&bscal[kBS_ABS], acc_abs);
-    lds_add(&bscal[kBS_W0], (double)n_pk * (double)P.w0);
+    if (lane == 0) {
+        lds_add(&bscal[kBS_W0], (double)n_pk * (double)P.w0);
+        lds_add(&bev[kBE_PACKETS], (unsigned long long)n_pk);
+    }
     lds_add(&bev[kBE_CROSS], (unsigned long long)n_cross);
-    lds_add(&bev[kBE_PACKETS], (unsigned long long)n_pk);
     lds_add(&bev[kBE_GLOBAL_DEPOSITS], (unsigned long long)n_glob);
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < kBS_COUNT; i += blockDim.x)

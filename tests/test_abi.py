@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), n
         assert n in _abi.SIGNATURES, n
-    assert L.pmo_abi_version() == _abi.ABI_VERSION == 4
+    assert L.pmo_abi_version() == _abi.ABI_VERSION == 5
     assert L.pmo_workspace_bytes() >= 8
 
 
@@ -39,7 +39,7 @@ PROBE = r"""
 int main(void) {
   S(pmo_volume) S(pmo_source) S(pmo_run) S(pmo_tallies) S(pmo_plate_instance) S(pmo_batch_tallies) S(pmo_diff) S(pmo_obd)
   O(pmo_diff, mismatch) O(pmo_plate_instance, n_packets) O(pmo_plate_instance, layers) O(pmo_run, key_dev) O(pmo_run, quorum)
-  O(pmo_volume, optics) O(pmo_source, w0) O(pmo_source, add) O(pmo_source, states_stride)
+  O(pmo_volume, optics) O(pmo_volume, skip) O(pmo_source, w0) O(pmo_source, add) O(pmo_source, states_stride)
   O(pmo_run, packets_per_lane) O(pmo_tallies, exit_state)
   O(pmo_tallies, ring_w) O(pmo_tallies, line_z) O(pmo_tallies, fluence_rz) O(pmo_run, flags) O(pmo_run, window)
   O(pmo_obd, stream) O(pmo_obd, side)
@@ -62,6 +62,7 @@ def test_ctypes_layout_matches_header(tmp_path):
                       ("pmo_diff", _abi.PmoDiff), ("pmo_obd", _abi.PmoObd)):
         assert int(out[name]) == C.sizeof(cls), name
     assert int(out["pmo_volume.optics"]) == _abi.PmoVolume.optics.offset
+    assert int(out["pmo_volume.skip"]) == _abi.PmoVolume.skip.offset
     assert int(out["pmo_source.w0"]) == _abi.PmoSource.w0.offset
     assert int(out["pmo_source.add"]) == _abi.PmoSource.add.offset
     assert int(out["pmo_source.states_stride"]) == _abi.PmoSource.states_stride.offset
@@ -111,6 +112,19 @@ def test_argument_errors_are_codes(field, value, msg):
     assert msg in L.pmo_last_error().decode()
     with pytest.raises(ValueError):
         _abi.check(rc)
+
+
+@pytest.mark.parametrize("args,msg", [
+    ((None, 5, 5, 5, 3, 1, 1), "non-null"), ((1, 5, 5, 5, 0, 1, 1), "cap"), ((1, 5, 5, 5, 128, 1, 1), "cap"),
+    ((1, 0, 5, 5, 3, 1, 1), "extents"), ((1, 2048, 1024, 1024, 3, 1, 1), "exceeds"),
+])
+def test_skip_radius_argument_errors(args, msg):
+    # checked on the host before any HIP call
+    L = _abi.lib()
+    lab, nx, ny, nz, cap, out, scratch = args
+    with pytest.raises((ValueError, _abi.PmoError), match=msg):
+        _abi.check(L.pmo_skip_radius(C.c_void_p(lab) if lab else None, nx, ny, nz, cap, C.c_void_p(out),
+                                     C.c_void_p(scratch), None))
 
 
 def test_philox_argument_errors():

@@ -24,7 +24,7 @@ def test_c_program_matches_the_python_layer():
     assert r.returncode == 0, r.stderr
     c = json.loads(r.stdout.strip().splitlines()[-1])
     wl = presets.slab_plate()
-    wk = Walker(wl.labels, wl.pitch, wl.optics)
+    wk = Walker(wl.labels, wl.pitch, wl.optics, skip=False)   # the C program passes no skip radius
     tal = wk.tallies(TallySpec(nr=50, dr=0.1))
     wk.walk(wl.source, N, seed, tal)
     torch.cuda.synchronize()
