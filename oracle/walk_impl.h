@@ -143,14 +143,15 @@ static void FN(record_exit)(const og_volume* V, const og_tallies* T, FN(ctx)* C,
     else if (v[2] > 0) C->scal[1] += wd;
     else C->scal[2] += wd;
     C->scal[7 + 2 * k + (s > 0 ? 1 : 0)] += wd;
-    int top = (k == 2 && s < 0);
+    /* top-face detectors: virtualOBD.py:165 selects v_z < 0 and z < 1e-8 on the encoded exit */
+    int top = (v[2] < 0 && z < 1e-8);
     double r = sqrt(x * x + y * y);
     if (T->nr > 0) {
         int b = FN(bin)(r, T->dr, T->nr);                   /* radialDistance, metrics/_scoring.py:16-25 */
         if (b >= 0) {
             if (v[2] < 0 && C->rd_r) C->rd_r[b] += wd;
             if (v[2] > 0 && C->tt_r) C->tt_r[b] += wd;
-            if (top && C->rd_top_r) C->rd_top_r[b] += wd;  /* virtualOBD.py:165 (v_z<0 & z<1e-8) */
+            if (top && C->rd_top_r) C->rd_top_r[b] += wd;
             if (v[2] < 0 && C->rd_wl) {
                 for (int j = 0; j < V->n_labels; j++) {
                     C->rd_wl[(int64_t)b * V->n_labels + j] += wd * Lj[j];

@@ -36,7 +36,7 @@ class DiffWalk:
     detected in `nr` bins of `dr` mm."""
 
     def __init__(self, labels, pitch, optics, source: Source, n_packets=1e7, nr=40, dr=0.1, device="cuda:0"):
-        self.walker = Walker(labels, pitch, optics, device=device)
+        self.walker = Walker(labels, pitch, optics, device=device, skip=False)   # replay steps voxel by voxel
         self.device = self.walker.device
         self.base_optics = np.array(optics, np.float32)
         self.n_labels = self.base_optics.shape[0] - 1

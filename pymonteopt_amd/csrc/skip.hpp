@@ -52,14 +52,17 @@ __global__ void __launch_bounds__(256) skip_pass_kernel(const uint8_t* __restric
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t c = (i / stride) % len;
         uint32_t m = in[i];
-        for (uint32_t d = 1u; d <= cap && d < m; ++d) {
-            if (c >= d) {
-                const uint32_t a = in[i - d * stride];
-                m = a > d ? (a < m ? a : m) : d;
-            }
-            if (c + d < len && d < m) {
-                const uint32_t a = in[i + d * stride];
-                m = a > d ? (a < m ? a : m) : d;
+        // four offsets per round, their eight loads independent of each other; a round
+        // starts only while an offset can still lower m (min-max with offset d is >= d)
+        for (uint32_t d0 = 1u; d0 <= cap && d0 < m; d0 += 4u) {
+#pragma unroll
+            for (uint32_t j = 0u; j < 4u; ++j) {
+                const uint32_t d = d0 + j;
+                const uint32_t lo = (d <= cap && c >= d) ? in[i - d * stride] : 255u;
+                const uint32_t hi = (d <= cap && c + d < len) ? in[i + d * stride] : 255u;
+                const uint32_t a = lo < hi ? lo : hi;
+                const uint32_t v = a > d ? a : d;
+                m = v < m ? v : m;
             }
         }
         out[i] = (uint8_t)(clamp && m > cap ? cap : m);

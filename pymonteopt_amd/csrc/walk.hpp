@@ -652,8 +652,12 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
                 const float fvz = (float)vz;
                 lds_add(&bscal[fvz < 0.f ? kBS_RD : (fvz > 0.f ? kBS_TT : kBS_OTHER)], wd);
                 lds_add(&bscal[kBS_FACE + 2 * xk + (xs > 0 ? 1 : 0)], wd);
-                const bool top = (xk == 2 && xs < 0);
-                const float r = __builtin_sqrtf(x * x + y * y);
+                // top-face detectors select v_z < 0 and z < 1e-8 on the encoded exit, the
+                // reference's selection (virtualOBD.py:165)
+                const bool top = fvz < 0.f && z < 1e-8f;
+                // squares and sum rounded one by one (no contraction), as numpy forms r from
+                // the float32 exit dump, so host and device binning part only at bin edges
+                const float r = __builtin_sqrtf(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y)));
                 if (P.nr > 0) {
                     const int b = bin_of(r, P.dr, P.nr);
                     if (b >= 0) {

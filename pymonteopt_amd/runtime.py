@@ -152,11 +152,13 @@ class Walker:
         self.skip = self.nvox > SKIP_MIN_VOXELS if skip is None else bool(skip)
         self.skip_cap = int(skip_cap)
         self._skip_radius = None
+        if self.skip:
+            self.skip_radius()          # part of the geometry upload, outside any timed walk
 
     def skip_radius(self) -> torch.Tensor:
         """The per-voxel uniform-cube radius (uint8, the label volume's layout), computed on
-        the device at first use; the computing stream is synchronised once, so walks on any
-        stream may read it."""
+        the device once; the computing stream is synchronised, so walks on any stream may
+        read it."""
         if self._skip_radius is None:
             out = torch.empty_like(self.labels)
             scratch = torch.empty_like(self.labels)

@@ -37,7 +37,7 @@ def test_batched_matches_plain_walks(layer_step):
     assert res["events"]["packets"] == sum(p.n_packets for p in ps)
     for i, p in enumerate(ps):
         kind, vox, w0 = p.source()
-        wk = Walker(p.volume(), p.voxel_space, p.optics())
+        wk = Walker(p.volume(), p.voxel_space, p.optics(), skip=False)   # voxel by voxel, as the sweep
         tal = wk.tallies(TallySpec(nr=20, dr=0.1, absorbed=False))
         wk.walk(Source(voxel=vox, w0=w0, kind=kind, radius=p.radius), p.n_packets, 5, tal, stream_id=40 + i)
         torch.cuda.synchronize()

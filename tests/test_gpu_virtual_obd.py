@@ -75,10 +75,20 @@ def test_device_detectors_match_the_exit_dump(tmp_path):
     res = model.start(11).get_result()
     dev = vo.detector_curves({k: res[k] for k in ("r", "z", "Rd_top_r", "Tt_r", "line_z", "nPh")})
     host = vo.detector_curves({"p": res["p"], "v": res["v"], "w": res["w"], "nPh": res["nPh"]})
+    n = res["nPh"]
+    i = np.arange(300)
+    areas = {"B": 2 * np.pi * (i + 0.5) * 0.1 ** 2, "F": 2 * np.pi * (i + 0.5) * 0.1 ** 2,
+             "L": np.full(400, 0.1 * 5 * 2)}
     for (ad, yd), (ah, yh), name in zip(dev, host, "BFL"):
         assert np.array_equal(ad, ah), name
-        assert np.allclose(yd, yh, rtol=1e-5, atol=1e-12), (name, np.abs(yd - yh).max())
         assert yd.sum() > 0, name
+        # in packet weight per bin: the same exits land in the same bins, except that the
+        # device bins r with a float32 width and the host against float64 edges, so an exit
+        # within a float32 ulp of an edge may fall in the neighbouring bin
+        wd, wh = yd * areas[name] * n, yh * areas[name] * n
+        assert abs(wd.sum() - wh.sum()) <= 1e-9 * wh.sum(), name
+        moved = np.abs(wd - wh) > 1e-9 * wh.sum()
+        assert moved.sum() <= 4, (name, np.flatnonzero(moved), (wd - wh)[moved])
 
 
 def test_overlapped_run_writes_the_sequential_files(tmp_path):
