@@ -21,6 +21,7 @@ the first dermis voxel of the centre column (VoxelTuringModel._set_inital_add, :
 from __future__ import annotations
 
 import numpy as np
+import torch
 
 from .geometry import center_add, optics_table
 
@@ -114,9 +115,16 @@ class _TuringBase:
             self.params["voxel_space"] = vs
             self.voxel_space = vs
         vm = np.ascontiguousarray(bone_model)
-        if vm.ndim != 3 or vm.min(initial=0) < 0 or vm.max(initial=0) > 1:
+        if vm.ndim != 3:
             raise ValueError("the bone model must be a 3-D 0/1 volume")
-        vm = vm.astype(np.uint8)
+        if vm.dtype in (np.int8, np.uint8, np.bool_):
+            vm = vm.view(np.uint8)           # no copy; a negative int8 reads as >= 128
+        elif vm.min(initial=0) < 0:
+            raise ValueError("the bone model must be a 3-D 0/1 volume")
+        else:
+            vm = vm.astype(np.uint8)
+        if vm.max(initial=0) > 1:
+            raise ValueError("the bone model must be a 3-D 0/1 volume")
         self.voxel_model = self._make_voxel_model(vm)
         self.getModelSize()
 
@@ -138,12 +146,31 @@ class TuringModel_Rectangular(_TuringBase):
         super().__init__()
 
     def _make_voxel_model(self, vm):
+        # the cropped core padded on x and z by cortical, subcutis and dermis, then by the end
+        # label on every axis (add_array, :585-601), written into one allocation: the band
+        # of each layer, then the core (a subject volume is ~1 GB, and each np.pad copies it)
         c = self._rect_crop(vm.shape[0])
-        vm = vm[c:-c, :, c:-c]
-        for lab, k in zip(self.layer_labels, self._layers()):
-            vm = _pad(vm, int(k), lab)
+        core = vm[c:-c, :, c:-c]
+        k = [int(t) for t in self._layers()]
+        K = sum(k)
+        X, Y, Z = core.shape[0] + 2 * K + 2, core.shape[1] + 2, core.shape[2] + 2 * K + 2
+        out = np.empty((X, Y, Z), np.uint8)
+        T = torch.from_numpy(out)            # torch's host copies and fills use every core
+        end = self.end_point
+        for v in (T[0], T[-1], T[:, 0], T[:, -1], T[:, :, 0], T[:, :, -1]):
+            v.fill_(end)
+        ys = slice(1, Y - 1)
+        o = 1
+        for lab, t in reversed(list(zip(self.layer_labels, k))):     # outermost layer first
+            if t > 0:
+                T[o:o + t, ys, o:Z - o].fill_(lab)
+                T[X - o - t:X - o, ys, o:Z - o].fill_(lab)
+                T[o + t:X - o - t, ys, o:o + t].fill_(lab)
+                T[o + t:X - o - t, ys, Z - o - t:Z - o].fill_(lab)
+            o += t
+        T[o:X - o, ys, o:Z - o].copy_(torch.from_numpy(core))       # core is a strided view of vm
         self.launch_z = 1
-        return _pad(vm, 1, self.end_point, y_axis=True)
+        return out
 
 
 class TuringModel_Cylinder(_TuringBase):

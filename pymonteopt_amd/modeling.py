@@ -168,10 +168,11 @@ class TuringPattern:
         with torch.cuda.device(self.device):
             _abi.check(_abi.lib().pmo_binarize(C.c_void_p(u.data_ptr()), u.numel(), C.c_void_p(labels.data_ptr()),
                                                C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)))
-        U = labels.cpu().numpy()
+        U = labels.cpu()
         if self.tile_num_xz != 0:
-            U = np.tile(U, (self.tile_num_xz, self.tile_num_y, self.tile_num_xz))
-        return U
+            # np.tile's layout, with torch's multi-threaded host copy
+            U = U.repeat(self.tile_num_xz, self.tile_num_y, self.tile_num_xz)
+        return U.numpy()
 
     def _save_dicom(self, field, path):
         """The field times ct_coef as 16-bit slices along the last axis, path/0000.dcm ...

@@ -84,3 +84,21 @@ def test_cylinder_shells(cls):
         assert col[m.launch_z] == 4 and np.all(col[1:m.launch_z] == 5)
     else:
         assert m.launch_z == 1
+
+
+@pytest.mark.parametrize("layers", [(0.2, 0.3, 0.4), (0.0, 0.3, 0.1), (0.1, 0.0, 0.0)])
+def test_rectangular_equals_successive_padding(layers):
+    """The one-allocation build equals the reference's sequence of add_array paddings
+    (:585-601): cortical, subcutis, dermis on x and z, then the end label on every axis."""
+    m = TuringModel_Rectangular()
+    m.set_params(voxel_space=0.1, xz_size=3.0, th_cortical=layers[0], th_subcutaneus=layers[1], th_dermis=layers[2])
+    u = bone(n=36, seed=3)
+    m.build(u)
+    k = np.round(np.array(layers) / 0.1).astype(int)
+    c = int(36 / 2 - round(3.0 / 0.2)) + int(k[0])
+    want = u[c:-c, :, c:-c].astype(np.uint8)
+    for lab, t in zip((2, 3, 4), k):
+        if t > 0:
+            want = np.pad(want, [(t, t), (0, 0), (t, t)], constant_values=lab)
+    want = np.pad(want, 1, constant_values=5)
+    assert np.array_equal(m.voxel_model, want)

@@ -53,6 +53,7 @@ def test_modeling_end_to_end(tmp_path):
     U = t.modeling(str(tmp_path))
     n = t.gridsize
     assert U.shape == (2 * n, n, 2 * n) and U.dtype == np.int8 and set(np.unique(U)) <= {0, 1}
+    assert np.array_equal(U[:n], U[n:]) and np.array_equal(U[:, :, :n], U[:, :, n:])   # np.tile's layout
     assert abs(U.mean() - t.bv_tv_real) < 1e-12
     info = json.load(open(tmp_path / "_info.json"))
     assert info["model_shape"] == [n, n, n] and info["microarchitecture"]["BV/TV real"] == pytest.approx(t.bv_tv_real)
