@@ -98,7 +98,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
     double g_len = 0.0, g_coef = 0.0;
     int32_t rec_bin = -1;
     float rec_w = 0.f;
-    uint32_t n_cross = 0, n_pk = 0;
+    uint32_t n_cross = 0;
+    uint32_t n_pk = 0;               // packets this wave launched (wave-uniform)
 
     unsigned long long b_next = 0, b_end = 0;
     bool exhausted = false;
@@ -156,6 +157,9 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
                 unsigned long long claim = 0;
                 if (lane == 0) claim = atomicAdd(P.work, (unsigned long long)P.batch);
                 claim = __shfl(claim, 0);
+                // wave-uniform: kept in scalar registers
+                claim = ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(claim >> 32)) << 32) |
+                        (unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)claim);
                 if (claim >= P.n_packets) {
                     exhausted = true;
                 } else {
@@ -177,7 +181,6 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
                         skip = (c == 0.0);
                         g_coef = -c * (double)rec_w;
                     }
-                    ++n_pk;
                     if (!skip) {
                         const uint32_t plo = (uint32_t)pid, phi = (uint32_t)(pid >> 32);
                         ax = P.ix; ay = P.iy; az = P.iz;
@@ -201,7 +204,9 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
                         alive = true;
                     }
                 }
-                b_next += (uint32_t)__popcll(need) < avail ? (uint32_t)__popcll(need) : avail;
+                const uint32_t launched = (uint32_t)__popcll(need) < avail ? (uint32_t)__popcll(need) : avail;
+                b_next += launched;
+                n_pk += launched;                 // wave-uniform
             }
         }
         if (__ballot(alive) == 0ull) break;
@@ -367,9 +372,11 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
 
     if (!replay) {
         lds_add(&bscal[kBS_ABS], acc_abs);
-        lds_add(&bscal[kBS_W0], (double)n_pk * (double)P.w0);
+        if (lane == 0) {
+            lds_add(&bscal[kBS_W0], (double)n_pk * (double)P.w0);
+            lds_add(&bev[kBE_PACKETS], (unsigned long long)n_pk);
+        }
         lds_add(&bev[kBE_CROSS], (unsigned long long)n_cross);
-        lds_add(&bev[kBE_PACKETS], (unsigned long long)n_pk);
     }
     __syncthreads();
     if (!replay) {

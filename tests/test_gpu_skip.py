@@ -104,3 +104,18 @@ def test_voxel_stepping_matches_oracle(slab_pair, oracle):
     O = [oracle_run(oracle, wl, N, s) for s in range(10)]
     for k in ("Rd", "Tt", "A"):
         assert zscore([g["scalars"][k] for g in G], [o["scalars"][k] for o in O]) < 3, k
+
+
+def test_default_skips_where_it_pays(torch_cuda):
+    """Walker(skip=None) skips on the plates and the bone phantom, not on the smallest volume
+    (12^3) nor on the fragmented 64^3 four-label grid (mean radius 0.24)."""
+    from pymonteopt_amd import presets
+    from pymonteopt_amd.runtime import Walker
+    cases = ((presets.slab_plate(), True), (presets.bone_workload(seed=0, n=128), True),
+             (presets.grf_grid(seed=0), False),
+             (presets.plate("slab 12^3", [5], [0.01], [10], [0.9], [1.5], 1.0, 0.5, 5, 5), False))
+    for wl, want in cases:
+        wk = Walker(wl.labels, wl.pitch, wl.optics)
+        assert wk.skip is want, wl.name
+        assert (wk._skip_radius is not None) is want, wl.name
+        assert Walker(wl.labels, wl.pitch, wl.optics, skip=True).skip
