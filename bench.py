@@ -606,7 +606,10 @@ def main():
         "unit": "atomics/s",
         "frac": glob_pp * N / mean_launch_s / peaks["fp64_atomics_per_s"],
         "global_deposits_per_packet": glob_pp,
+        "peak_source": "libpmo's device-scope FP64 atomic-add probe (no published figure exists)",
     }
+    # the resource that binds this walk, measured: the FP64 atomics of the absorbed map
+    out["roofline"]["binding"] = "roofline_atomic"
     prof = traffic_from_profile()
     if prof is not None and int(prof.get("packets_per_launch", 0)) > 0:
         scale = N / float(prof["packets_per_launch"])
