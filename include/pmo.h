@@ -253,8 +253,8 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run,
 #define PMO_SKIP_MAX_CAP 127
 
 /* radius[v] = min(cap, Chebyshev distance from voxel v to the nearest voxel that lies on the
- * volume's outer face or has a 26-neighbour of another label): the cube of half-width
- * radius[v] voxels around v holds v's label only. labels is the walk's volume; radius and
+ * volume's outer face, has a 26-neighbour of another label or holds label 255): the cube of
+ * half-width radius[v] voxels around v holds v's label only. labels is the walk's volume; radius and
  * scratch are device buffers of nx*ny*nz bytes owned by the caller (scratch is overwritten).
  * 1 <= cap <= PMO_SKIP_MAX_CAP. Feeds pmo_volume.skip; no reference counterpart: the
  * reference steps every voxel face (pymopt/voxel/_kernel.py:62-125). */

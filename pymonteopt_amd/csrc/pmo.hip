@@ -359,16 +359,32 @@ int pmo_skip_radius(const uint8_t* labels, int32_t nx, int32_t ny, int32_t nz, i
     if (cap < 1 || cap > PMO_SKIP_MAX_CAP) return fail(PMO_ERR_INVALID, "cap %d outside [1, %d]", cap, PMO_SKIP_MAX_CAP);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const uint32_t un = (uint32_t)n, c = (uint32_t)cap;
+    const uint32_t X = (uint32_t)nx, Y = (uint32_t)ny, Z = (uint32_t)nz;
     const unsigned grid = (unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
-    // b -> scratch; min-max along z -> radius; along y -> scratch; along x -> radius (clamped)
-    hipLaunchKernelGGL(pmo::skip_seed_kernel, dim3(grid), dim3(256), 0, s, labels, (uint32_t)nx, (uint32_t)ny,
-                       (uint32_t)nz, c, scratch);
-    hipLaunchKernelGGL(pmo::skip_pass_kernel, dim3(grid), dim3(256), 0, s, (const uint8_t*)scratch, radius, un,
-                       (uint32_t)nz, 1u, c, 0u);
-    hipLaunchKernelGGL(pmo::skip_pass_kernel, dim3(grid), dim3(256), 0, s, (const uint8_t*)radius, scratch, un,
-                       (uint32_t)ny, (uint32_t)nz, c, 0u);
-    hipLaunchKernelGGL(pmo::skip_pass_kernel, dim3(grid), dim3(256), 0, s, (const uint8_t*)scratch, radius, un,
-                       (uint32_t)nx, (uint32_t)ny * (uint32_t)nz, c, 1u);
+    const uint32_t bs = pmo::kSkipSweepBlock;
+    // a sweep along an axis of length len and stride S over n_outer outer slices, in launches
+    // of at most 65535 slices (the grid's y limit)
+    auto sweep = [&](bool seed, const uint8_t* in, uint8_t* out, uint32_t n_outer, uint32_t len, uint32_t S,
+                     uint32_t flag) {
+        for (uint32_t o = 0; o < n_outer; o += 65535u) {
+            const uint32_t m = n_outer - o < 65535u ? n_outer - o : 65535u;
+            const dim3 g((S + bs - 1) / bs, m);
+            if (seed)
+                hipLaunchKernelGGL(pmo::skip_seed_sweep_kernel, g, dim3(bs), 0, s, in, out, o, len, S, c, flag);
+            else
+                hipLaunchKernelGGL(pmo::skip_minmax_sweep_kernel, g, dim3(bs), (size_t)(2u * c + 1u) * bs, s, in, out,
+                                   o, len, S, c, flag);
+        }
+    };
+    // B: w_z -> scratch; w_zy -> radius; b (from w_zyx) -> scratch
+    hipLaunchKernelGGL(pmo::skip_seed_z_kernel, dim3(grid), dim3(256), 0, s, labels, un, Z, scratch);
+    sweep(true, scratch, radius, X, Y, Z, 0u);
+    sweep(true, radius, scratch, 1u, X, Y * Z, 1u);
+    // min-max along z -> radius; along y -> scratch; along x -> radius (clamped)
+    hipLaunchKernelGGL(pmo::skip_pass_kernel, dim3(grid), dim3(256), 0, s, (const uint8_t*)scratch, radius, un, Z, 1u,
+                       c, 0u);
+    sweep(false, radius, scratch, X, Y, Z, 0u);
+    sweep(false, scratch, radius, 1u, X, Y * Z, 1u);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "skip radius launch");
     return PMO_OK;

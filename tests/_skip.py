@@ -6,7 +6,8 @@ import numpy as np
 
 
 def boundary(labels):
-    """True on the outer face and where one of the 26 neighbours holds another label."""
+    """True on the outer face, where one of the 26 neighbours holds another label, and on
+    label 255 (the device's marker of a mixed neighbourhood)."""
     lab = np.asarray(labels)
     b = np.ones(lab.shape, bool)
     c = lab[1:-1, 1:-1, 1:-1]
@@ -17,7 +18,7 @@ def boundary(labels):
             for dz in (-1, 0, 1):
                 inner |= lab[1 + dx:nx - 1 + dx, 1 + dy:ny - 1 + dy, 1 + dz:nz - 1 + dz] != c
     b[1:-1, 1:-1, 1:-1] = inner
-    return b
+    return b | (lab == 255)
 
 
 def skip_radius(labels, cap):
@@ -41,7 +42,7 @@ def skip_radius(labels, cap):
 
 def skip_radius_brute(labels, cap):
     """The largest k <= cap whose cube of half-width k around the voxel lies inside the
-    volume and holds one label (0 on the outer face)."""
+    volume and holds one label (0 on the outer face and on label 255)."""
     lab = np.asarray(labels)
     out = np.zeros(lab.shape, np.uint8)
     nx, ny, nz = lab.shape
@@ -49,7 +50,7 @@ def skip_radius_brute(labels, cap):
         for y in range(ny):
             for z in range(nz):
                 k = 0
-                while k < cap:
+                while k < cap and lab[x, y, z] != 255:
                     j = k + 1
                     if x - j < 0 or y - j < 0 or z - j < 0 or x + j >= nx or y + j >= ny or z + j >= nz:
                         break

@@ -39,6 +39,15 @@ def test_device_radius_equals_host(torch_cuda, shape, n_labels, cap, seed):
     assert np.array_equal(device_radius(torch_cuda, lab, cap), skip_radius(lab, cap))
 
 
+def test_device_radius_with_label_255_and_long_axes(torch_cuda):
+    lab = blob_volume((7, 9, 300), 2, 4)
+    lab[:, 5:, 100:180] = 255
+    for cap in (15, 127):
+        assert np.array_equal(device_radius(torch_cuda, lab, cap), skip_radius(lab, cap)), cap
+    lab = blob_volume((300, 6, 5), 3, 6)
+    assert np.array_equal(device_radius(torch_cuda, lab, 40), skip_radius(lab, 40))
+
+
 def test_device_radius_on_model_volumes(torch_cuda):
     from pymonteopt_amd import presets
     for wl in (presets.skin_plate(), presets.bone_workload(seed=1, n=128)):

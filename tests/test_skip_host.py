@@ -42,3 +42,17 @@ def test_plate_radius_closed_form():
     want[d == 0] = 0
     assert np.array_equal(skip_radius(vol, 4), want)
     assert boundary(vol)[1, 5, 5] and not boundary(vol)[2, 5, 5]
+
+
+def test_label_255_is_boundary_and_leaves_other_radii():
+    """Label 255 (only ever the end label of a 255-label volume) counts as boundary; the
+    radii of the other voxels are those of the distance to their nearest other label."""
+    lab = blob_volume((14, 12, 13), 2, 5)
+    lab[:, :, 9:] = 255
+    r = skip_radius(lab, 15)
+    assert np.array_equal(r, skip_radius_brute(lab, 15))
+    assert np.all(r[lab == 255] == 0)
+    other = lab.copy()
+    other[other == 255] = 7
+    keep = lab != 255
+    assert np.array_equal(r[keep], skip_radius(other, 15)[keep])
