@@ -46,6 +46,36 @@ def cpu_baseline(wl, budget_s=15.0, max_packets=50_000_000):
             "sample": f"{n_s} packets of the same workload (f32 oracle, absorbed + Rd(r) tallies), {dt:.1f} s"}
 
 
+def cpu_baseline_batched(n_inst=512, budget_s=10.0, fraction=0.02):
+    """The oracle (f32 state, all host cores) walking the sweep's plates one after another,
+    each with `fraction` of its budget, in sweep order until about budget_s seconds: the
+    sequential CPU sweep at a reduced budget, as photons/s."""
+    import numpy as np
+    from oracle import oracle as og
+    from pymonteopt_amd.batched import slab_sweep
+    og.build()
+    threads = len(os.sched_getaffinity(0))
+    plates = slab_sweep(n_inst, seed=0)
+    done, t, k = 0, 0.0, 0
+    for p in plates:
+        kind, vox, w0 = p.source()
+        opt = p.optics().astype(np.float64)
+        L = opt.shape[0] - 1
+        n = max(1000, int(p.n_packets * fraction))
+        src = dict(kind=kind, voxel=tuple(int(x) for x in vox), w0=float(w0), radius=0.0, dir=(0.0, 0.0, 1.0))
+        t0 = time.perf_counter()
+        og.walk(p.volume(), p.voxel_space, opt[:L, 0], opt[:L, 1], opt[:L, 2], opt[:, 3], src, n, 2,
+                precision="f32", threads=threads, absorbed=False, nr=40, dr=0.1)
+        t += time.perf_counter() - t0
+        done += n
+        k += 1
+        if t >= budget_s:
+            break
+    return {"value": done / t, "unit": "photons/s", "cores": threads, "kind": "port",
+            "sample": f"the first {k} of the {n_inst} plates, {fraction:.0%} of each budget ({done} packets, "
+                      f"f32 oracle, Rd(r) tallies), {t:.1f} s"}
+
+
 def _events():
     import torch
     return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -633,6 +663,8 @@ def main():
                 out["obd"]["gpu_vs_cpu"] = out["obd"]["traces_per_s"] / out["obd"]["cpu_baseline"]["value"]
                 out["turing"]["cpu_baseline"] = cpu_baseline_turing()
                 out["published_slab"]["cpu_anchor"] = cpu_anchor_published_slab()
+                out["batched"]["cpu_baseline"] = cpu_baseline_batched()
+                out["batched"]["gpu_vs_cpu"] = out["batched"]["batched_photons_per_s"] / out["batched"]["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
 
 
