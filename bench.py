@@ -42,8 +42,23 @@ def cpu_baseline(wl, budget_s=15.0, max_packets=50_000_000):
             break
         n_s = int(min(max_packets, n_s * min(64.0, budget_s / dt)))
         seed += 1
+    # the CPU-runnable slab (101 x 101 x 50 @0.1 mm, mu_a 0.1, mu_s 10, g 0.9, n 1.37): its
+    # 1e5-packet seed batches, as many as fit in about 3 s
+    from pymonteopt_amd import presets
+    slab = presets.slab_plate()
+    sma, sms, sg, sn = slab.optics_columns()
+    n1, t1, k1 = 0, 0.0, 0
+    while t1 < 3.0 and k1 < 10:
+        t0 = time.perf_counter()
+        og.walk(slab.labels, slab.pitch, sma, sms, sg, sn, slab.source.as_dict(), 100_000, k1, precision="f32",
+                threads=threads, nr=50, dr=0.1)
+        t1 += time.perf_counter() - t0
+        n1 += 100_000
+        k1 += 1
     return {"value": n_s / dt, "unit": "photons/s", "cores": threads, "kind": "port",
-            "sample": f"{n_s} packets of the same workload (f32 oracle, absorbed + Rd(r) tallies), {dt:.1f} s"}
+            "sample": f"{n_s} packets of the same workload (f32 oracle, absorbed + Rd(r) tallies), {dt:.1f} s",
+            "slab_101x101x50": {"value": n1 / t1, "unit": "photons/s",
+                                "sample": f"seeds 0-{k1 - 1} x 1e5 packets (f32 oracle, absorbed + Rd(r)), {t1:.1f} s"}}
 
 
 def cpu_baseline_batched(n_inst=512, budget_s=10.0, fraction=0.02):
