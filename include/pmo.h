@@ -255,7 +255,8 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run,
 /* radius[v] = min(cap, Chebyshev distance from voxel v to the nearest voxel that lies on the
  * volume's outer face, has a 26-neighbour of another label or holds label 255): the cube of
  * half-width radius[v] voxels around v holds v's label only. labels is the walk's volume; radius and
- * scratch are device buffers of nx*ny*nz bytes owned by the caller (scratch is overwritten).
+ * scratch are distinct device buffers of nx*ny*nz bytes owned by the caller (scratch is
+ * overwritten).
  * 1 <= cap <= PMO_SKIP_MAX_CAP. Feeds pmo_volume.skip; no reference counterpart: the
  * reference steps every voxel face (pymopt/voxel/_kernel.py:62-125). */
 int pmo_skip_radius(const uint8_t* labels, int32_t nx, int32_t ny, int32_t nz, int32_t cap, uint8_t* radius,

@@ -353,6 +353,9 @@ int pmo_skip_radius(const uint8_t* labels, int32_t nx, int32_t ny, int32_t nz, i
 {
     g_err[0] = '\0';
     if (!labels || !radius || !scratch) return fail(PMO_ERR_INVALID, "labels, radius and scratch must be non-null");
+    if ((const void*)radius == (const void*)scratch || (const void*)labels == (const void*)radius ||
+        (const void*)labels == (const void*)scratch)
+        return fail(PMO_ERR_INVALID, "labels, radius and scratch must be distinct buffers");
     if (nx < 1 || ny < 1 || nz < 1) return fail(PMO_ERR_INVALID, "volume extents must be >= 1");
     const unsigned long long n = (unsigned long long)nx * ny * nz;
     if (n >= (1ull << 31)) return fail(PMO_ERR_UNSUPPORTED, "volume of %llu voxels exceeds 2^31", n);

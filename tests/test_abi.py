@@ -115,8 +115,9 @@ def test_argument_errors_are_codes(field, value, msg):
 
 
 @pytest.mark.parametrize("args,msg", [
-    ((None, 5, 5, 5, 3, 1, 1), "non-null"), ((1, 5, 5, 5, 0, 1, 1), "cap"), ((1, 5, 5, 5, 128, 1, 1), "cap"),
-    ((1, 0, 5, 5, 3, 1, 1), "extents"), ((1, 2048, 1024, 1024, 3, 1, 1), "exceeds"),
+    ((None, 5, 5, 5, 3, 2, 3), "non-null"), ((1, 5, 5, 5, 0, 2, 3), "cap"), ((1, 5, 5, 5, 128, 2, 3), "cap"),
+    ((1, 0, 5, 5, 3, 2, 3), "extents"), ((1, 2048, 1024, 1024, 3, 2, 3), "exceeds"),
+    ((1, 5, 5, 5, 3, 2, 2), "distinct"), ((1, 5, 5, 5, 3, 1, 2), "distinct"),
 ])
 def test_skip_radius_argument_errors(args, msg):
     # checked on the host before any HIP call
