@@ -120,14 +120,17 @@ def optics_table(ma, ms, g, n, n_ambient) -> np.ndarray:
     return t
 
 
-def validate_volume(vol: np.ndarray, n_labels: int):
-    """Checks the contract of the walk: uint8, C order, labels <= n_labels, end-label shell."""
+def validate_volume(vol: np.ndarray, n_labels: int, shell: bool = True):
+    """Checks the contract of the walk: uint8, C order, labels <= n_labels, end-label shell
+    (shell=False: the type, shape and n_labels checks only)."""
     if vol.dtype != np.uint8 or vol.ndim != 3:
         raise ValueError("label volume must be a 3-D uint8 array")
     if min(vol.shape) < 3:
         raise ValueError("label volume needs at least 3 voxels per axis (1-voxel end shell)")
     if not (1 <= n_labels <= 255):
         raise ValueError("n_labels must be in [1, 255]")
+    if not shell:
+        return
     if int(vol.max()) > n_labels:
         raise ValueError(f"label {int(vol.max())} exceeds the end label {n_labels}")
     end = n_labels

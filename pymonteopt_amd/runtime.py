@@ -140,7 +140,9 @@ class Walker:
             raise ValueError("optics must be float32 [n_labels+1][4]")
         self.n_labels = optics.shape[0] - 1
         if validate:
-            validate_volume(labels, self.n_labels)
+            # the checks of validate_volume; the full scans (largest label, the end shell) run
+            # on the device after the upload
+            validate_volume(labels, self.n_labels, shell=False)
         self.shape = tuple(int(s) for s in labels.shape)
         self.pitch = float(pitch)
         self.device = torch.device(device)
@@ -148,6 +150,8 @@ class Walker:
             raise RuntimeError("the walk runs on a HIP device only (no CPU fallback)")
         self.lib = _abi.lib()
         self.labels = torch.from_numpy(labels.reshape(-1)).to(self.device)
+        if validate:
+            self._validate_on_device()
         self.optics = torch.from_numpy(optics).to(self.device)
         self.workspace = torch.zeros(int(self.lib.pmo_workspace_bytes()), dtype=torch.uint8,
                                      device=self.device)
@@ -176,6 +180,17 @@ class Walker:
                 st.synchronize()
             self._skip_radius = out
         return self._skip_radius
+
+    def _validate_on_device(self):
+        v = self.labels.view(self.shape)
+        end = self.n_labels
+        top = int(v.max())
+        if top > end:
+            raise ValueError(f"label {top} exceeds the end label {end}")
+        faces = (v[0], v[-1], v[:, 0], v[:, -1], v[:, :, 0], v[:, :, -1])
+        if not bool(torch.stack([(f != end).any() for f in faces]).any()):
+            return
+        raise ValueError("the outer 1-voxel shell of the volume must hold the end label")
 
     @property
     def nvox(self):

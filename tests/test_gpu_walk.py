@@ -483,3 +483,23 @@ def test_packet_range_shards_sum_to_the_whole_run(slab):
         assert parts["scalars"][k] == pytest.approx(whole["scalars"][k], rel=1e-12), k
     assert np.allclose(parts["absorbed"], whole["absorbed"], rtol=1e-10, atol=1e-15)
     assert np.allclose(parts["rd_r"], whole["rd_r"], rtol=1e-10, atol=0)
+
+
+def test_walker_checks_the_volume_on_the_device(torch_cuda):
+    """The Walker's scans of the label volume (largest label, end-label shell) run on the
+    device after the upload and raise as the host check does."""
+    from pymonteopt_amd import presets
+    from pymonteopt_amd.runtime import Walker
+    wl = presets.slab_plate()
+    Walker(wl.labels, wl.pitch, wl.optics)
+    bad = wl.labels.copy()
+    bad[50, 50, 20] = 7
+    with pytest.raises(ValueError, match="exceeds the end label"):
+        Walker(bad, wl.pitch, wl.optics)
+    for face in ((0, slice(None), slice(None)), (slice(None), -1, slice(None)), (slice(None), slice(None), 0)):
+        bad = wl.labels.copy()
+        bad[face][3] = 0
+        with pytest.raises(ValueError, match="outer 1-voxel shell"):
+            Walker(bad, wl.pitch, wl.optics)
+    with pytest.raises(ValueError, match="3-D uint8"):
+        Walker(wl.labels.astype(np.int32), wl.pitch, wl.optics)
