@@ -45,7 +45,7 @@ extern "C" {
 #define PMO_ERR_INVALID (-1)     /* bad argument: null pointer, size or range        */
 #define PMO_ERR_LABEL (-2)       /* label volume / optics table inconsistency         */
 #define PMO_ERR_HIP (-3)         /* HIP runtime error (launch, device query)          */
-#define PMO_ERR_UNSUPPORTED (-4) /* request outside what this build implements        */
+#define PMO_ERR_UNSUPPORTED (-4) /* request outside what this library implements      */
 
 /* ---- beam kinds ------------------------------------------------------------------ */
 #define PMO_BEAM_PENCIL 0 /* all packets start at the centre of the launch voxel's top face  */

@@ -3,8 +3,8 @@
 The reference writes each z-slice of a Turing-pattern volume as one DICOM file
 (TuringPattern._write_dicom, pymopt/modeling_gpu/_classes.py:185-222). It reads such stacks
 back in TuringModel_*._read_dicom (pymopt/voxel/_classes.py:577-586) and in
-`readDicom` (pymopt/utils/readDICOM.py:17-44). Both go through pydicom, which this build
-does not include.
+`readDicom` (pymopt/utils/readDICOM.py:17-44). Both go through pydicom, which this package
+does not use.
 
 This module reads and writes the DICOM subset those files use:
 * a Part-10 file: a 128-byte preamble, "DICM", then the file-meta group in explicit-VR

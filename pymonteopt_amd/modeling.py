@@ -16,7 +16,7 @@ Differences:
 * Reaction terms are the reference's family f = u + dt (a1 u + a2 v + a3 u^3) and
   g = v + dt (b1 u + b2 v). `set_activator_function` and `set_inhibitor_function` accept a
   callable of that form and recover its coefficients; any other form raises ValueError.
-* DICOM output (`save_dicom=True`) is written by `pymonteopt_amd.dicom`, this build's own
+* DICOM output (`save_dicom=True`) is written by `pymonteopt_amd.dicom`, this package's own
   reader and writer of uncompressed little-endian files, because pydicom is absent.
 """
 from __future__ import annotations

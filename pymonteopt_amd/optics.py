@@ -135,7 +135,7 @@ class OBD:
 
     def set_monte_data(self, *initial_data, **kwargs):
         """Exit states p [3][n], v [3][n], w [n] and nPh. A dict argument may be a model's
-        whole get_result(): only these keys are taken from it (this build's result dict
+        whole get_result(): only these keys are taken from it (this package's result dict
         also holds the tallies). Keywords must be known keys (KeyError otherwise)."""
         dicts = [{k: d[k] for k in d if k in self.keys_data} for d in initial_data]
         set_params(self.data, self.keys_data, *dicts, **kwargs)
