@@ -22,6 +22,18 @@ sys.path.insert(0, ROOT)
 METRIC = "photons/s, 3-layer skin 1e8 ph; batched & adjoint vs plain, same GPU, 3σ"
 
 
+def _cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or "unknown"."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(wl, budget_s=15.0, max_packets=50_000_000):
     """The oracle (C restatement of the reference walk, float32 state like the reference's
     float32 arrays) on all host cores, on a bounded sample of the same workload: a probe
@@ -55,7 +67,7 @@ def cpu_baseline(wl, budget_s=15.0, max_packets=50_000_000):
         t1 += time.perf_counter() - t0
         n1 += 100_000
         k1 += 1
-    return {"value": n_s / dt, "unit": "photons/s", "cores": threads, "kind": "port",
+    return {"value": n_s / dt, "unit": "photons/s", "cores": threads, "kind": "port", "cpu_model": _cpu_model(),
             "sample": f"{n_s} packets of the same workload (f32 oracle, absorbed + Rd(r) tallies), {dt:.1f} s",
             "slab_101x101x50": {"value": n1 / t1, "unit": "photons/s",
                                 "sample": f"seeds 0-{k1 - 1} x 1e5 packets (f32 oracle, absorbed + Rd(r)), {t1:.1f} s"}}
