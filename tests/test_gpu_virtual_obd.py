@@ -112,3 +112,36 @@ def test_overlapped_run_writes_the_sequential_files(tmp_path):
     for k, v in out[False].items():
         assert np.allclose(out[True][k], v, rtol=1e-12, atol=0), k
         assert v.sum() > 0, k
+
+
+def test_subject_walk_matches_oracle(tmp_path, oracle):
+    """The default walk (same-label faces crossed in one step) through a reduced subject --
+    dermis, subcutis, cortical shell and the Turing trabecular core, with their index steps
+    -- against the oracle: 10 seed batches each on independent streams, Rd, Tt and A within
+    3 pooled SE; the energy ledger to 1e-6."""
+    from _parity import ledger_error, zscore
+    from pymonteopt_amd import virtual_obd as vo
+    from pymonteopt_amd.runtime import TallySpec
+    from pymonteopt_amd.voxel import VoxelTuringModel
+    mp, mo = _params()
+    vp = _subject(4)
+    u, _ = vo.generate_bone_model(vp["bv_tv"][0], str(tmp_path / "model"), mp)
+    vo.subject_params(vp, mo)
+    model = VoxelTuringModel(nPh=1000, model_name="TuringModel", nr=0, fluence=False)
+    model.set_model(u)
+    model.build(**{k: v for k, v in mo.items() if k not in ("symmetrization", "enclosure")})
+    wk, src = model.walker(), model.source()
+    assert wk.skip
+    N = 20000
+    G = []
+    for s in range(10):
+        tal = wk.tallies(TallySpec(nr=0, absorbed=False))
+        wk.walk(src, N, 100 + s, tal)
+        G.append(tal.to_host()["scalars"])
+        assert ledger_error(G[-1]) < 1e-6
+    vm, opt = model.model.voxel_model, model.model.optics().astype(np.float64)
+    L = opt.shape[0] - 1
+    O = [oracle.walk(vm, float(model.model.voxel_space), opt[:L, 0], opt[:L, 1], opt[:L, 2], opt[:, 3],
+                     src.as_dict(), N, s, absorbed=False)["scalars"] for s in range(10)]
+    for k in ("Rd", "Tt", "A"):
+        assert zscore([g[k] for g in G], [o[k] for o in O]) < 3, k
