@@ -106,7 +106,7 @@ typedef struct pmo_volume {
                               entry n_labels holds n_ambient (mu_a, mu_s, g unused)            */
     const uint8_t* skip;   /* device [nx][ny][nz], pmo_skip_radius of these labels, or NULL.
                               With it pmo_walk crosses the faces inside same-label regions in
-                              one step (the same draws and events; merged arithmetic); NULL
+                              one step (the same draws; merged arithmetic, rounded once); NULL
                               steps voxel by voxel as the reference does. Other entry points
                               ignore it.                                                       */
 } pmo_volume;

@@ -325,8 +325,8 @@ __device__ __forceinline__ int32_t clamp_index(int32_t a, uint32_t n)
 // voxel of radius r > 0 treats the cube of half-width r around it as one voxel: one step
 // reaches its face or the hop's end. The faces inside the cube join voxels of one label,
 // where the reference moves on without a draw (_kernel.py:82), so every packet makes the
-// same draws and the same events; only the arithmetic of the merged steps is rounded once
-// instead of per face.
+// same draws (and events, unless rounding flips a decision at a face); only the arithmetic
+// of the merged steps is rounded once instead of per face.
 template <typename R, bool RZ, bool SKIP>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) walk_kernel(const WalkParams P)
 {

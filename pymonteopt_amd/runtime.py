@@ -132,8 +132,8 @@ class Walker:
         """skip: cross the faces inside same-label regions in one step (pmo_skip_radius,
         radius capped at skip_cap voxels); False steps voxel by voxel as the reference does;
         None (default) skips on volumes of more than 16*16*8 voxels whose mean radius is at
-        least SKIP_MIN_MEAN_RADIUS. Either way every packet makes the same draws and the same
-        events."""
+        least SKIP_MIN_MEAN_RADIUS. Either way every packet makes the same draws, and the same
+        events unless rounding flips a decision at a face."""
         labels = np.ascontiguousarray(labels)
         optics = np.ascontiguousarray(optics, dtype=np.float32)
         if optics.ndim != 2 or optics.shape[1] != 4:
