@@ -162,7 +162,8 @@ class Walker:
         self._skip_radius = None
         if self.skip:
             r = self.skip_radius()      # part of the geometry upload, outside any timed walk
-            if skip is None and int(r.sum(dtype=torch.int64)) < SKIP_MIN_MEAN_RADIUS * self.nvox:
+            # an integer tensor's sum accumulates in int64 (an explicit dtype would cast a copy)
+            if skip is None and int(r.sum()) < SKIP_MIN_MEAN_RADIUS * self.nvox:
                 self.skip = False       # too fragmented to gain
                 self._skip_radius = None
 
