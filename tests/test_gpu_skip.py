@@ -128,3 +128,28 @@ def test_default_skips_where_it_pays(torch_cuda):
         assert wk.skip is want, wl.name
         assert (wk._skip_radius is not None) is want, wl.name
         assert Walker(wl.labels, wl.pitch, wl.optics, skip=True).skip
+
+
+def test_skipping_on_the_bone_phantom(torch_cuda):
+    """On the 128^3 bone phantom (bone/marrow index steps at every trabecula) the skipping
+    and the voxel-stepping walks, same seeds: the same packet counts, Rd, Tt and A within a
+    fraction of the seed-to-seed spread, and the energy ledger."""
+    from pymonteopt_amd import presets
+    from pymonteopt_amd.runtime import Walker
+    wl = presets.bone_workload(seed=2, n=128)
+    ws = Walker(wl.labels, wl.pitch, wl.optics, skip=True)
+    wv = Walker(wl.labels, wl.pitch, wl.optics, skip=False)
+    N = 100000
+    S = [gpu_run(ws, wl, N, s) for s in range(6)]
+    V = [gpu_run(wv, wl, N, s) for s in range(6)]
+    for a, b in zip(S, V):
+        assert ledger_error(a["scalars"]) < 1e-6
+        assert a["events"]["packets"] == b["events"]["packets"] == N
+        for k in ("interactions", "crossings", "fresnel"):
+            assert abs(a["events"][k] - b["events"][k]) <= 2e-3 * b["events"][k], (k, a["events"][k], b["events"][k])
+    for k in ("Rd", "Tt", "A"):
+        x = np.array([r["scalars"][k] / N for r in S])
+        y = np.array([r["scalars"][k] / N for r in V])
+        spread = np.std(y, ddof=1)
+        print(k, "skip-voxel", np.max(np.abs(x - y)) / spread, flush=True)
+        assert np.max(np.abs(x - y)) < 0.5 * spread, (k, x - y, spread)
