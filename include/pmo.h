@@ -249,6 +249,17 @@ int pmo_device_info(int32_t* n_cu, int32_t* blocks_per_cu_walk, int32_t block_si
 int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run,
              const pmo_tallies* tal, void* workspace, void* stream);
 
+/* pmo_walk without a workspace argument, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. Its scratch, the walk's 256 B packet-claim counter block, is
+ * a static device array of libpmo.so (one per GPU, held until the process ends); that need is
+ * the same at every volume and packet count, so no size is refused for it. The call makes no
+ * hipMalloc/hipFree; the counters are cleared by a kernel on `stream`. Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (two streams, two threads) race and
+ * must be serialised by the caller. Arguments, results and errors are pmo_walk's. */
+int pmo_walk_noalloc(const pmo_volume* vol, const pmo_source* src, const pmo_run* run,
+                     const pmo_tallies* tal, void* stream);
+
 /* Largest radius pmo_skip_radius accepts */
 #define PMO_SKIP_MAX_CAP 127
 

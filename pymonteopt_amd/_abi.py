@@ -115,6 +115,8 @@ SIGNATURES = {
     "pmo_device_info": ([C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32], C.c_int),
     "pmo_walk": ([C.POINTER(PmoVolume), C.POINTER(PmoSource), C.POINTER(PmoRun), C.POINTER(PmoTallies),
                   C.c_void_p, C.c_void_p], C.c_int),
+    "pmo_walk_noalloc": ([C.POINTER(PmoVolume), C.POINTER(PmoSource), C.POINTER(PmoRun), C.POINTER(PmoTallies),
+                          C.c_void_p], C.c_int),
     "pmo_philox": ([C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
     "pmo_skip_radius": ([C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                          C.c_void_p], C.c_int),

@@ -110,6 +110,37 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 }  // namespace
 
+// Scratch of pmo_walk_noalloc: the walk's packet-claim counters in a static device array,
+// so that entry point allocates nothing. Each device holds its own copy at its own
+// address, looked up once per device with that device current; host code uses that
+// address, never the array's name (a host-side stand-in).
+static __device__ __align__(256) unsigned long long scratch[kWorkspaceBytes / sizeof(unsigned long long)];
+
+namespace {
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64];
+
+int scratch_address(void** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= 64) return fail(PMO_ERR_UNSUPPORTED, "device ordinal %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!p) return fail(PMO_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return PMO_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int pmo_abi_version(void) { return PMO_ABI_VERSION; }
@@ -346,6 +377,20 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "walk launch");
     return PMO_OK;
+}
+
+int pmo_walk_noalloc(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, const pmo_tallies* tal,
+                     void* stream)
+{
+    g_err[0] = '\0';
+    // the walk's scratch is its packet-claim counter block, the same 256 B at every size
+    if (pmo_workspace_bytes() > sizeof(scratch))
+        return fail(PMO_ERR_UNSUPPORTED, "the walk needs %zu B of scratch, the static scratch holds %zu B",
+                    pmo_workspace_bytes(), sizeof(scratch));
+    void* ws = nullptr;
+    int rc = scratch_address(&ws);
+    if (rc != PMO_OK) return rc;
+    return pmo_walk(vol, src, run, tal, ws, stream);
 }
 
 int pmo_skip_radius(const uint8_t* labels, int32_t nx, int32_t ny, int32_t nz, int32_t cap, uint8_t* radius,

@@ -231,10 +231,11 @@ class Walker:
             raise ValueError(f"launch states must be add, p, v of shape (3, N) and w of shape (N,), N >= {n}")
         return out
 
-    def walk(self, src: Source, n_packets: int, seed: int, tallies: TallySet, *, stream_id=0,
-             first_packet=0, wth=1e-4, roulette_m=10.0, fp64=False, normal_fresnel=False,
-             block_size=0, grid_blocks=0, packets_per_lane=0, window=(0, 0, 0), quorum=0, hip_stream=None):
-        """Launch one walk (asynchronous on torch's current stream unless given)."""
+    def c_args(self, src: Source, n_packets: int, seed: int, tallies: TallySet, *, stream_id=0,
+               first_packet=0, wth=1e-4, roulette_m=10.0, fp64=False, normal_fresnel=False,
+               block_size=0, grid_blocks=0, packets_per_lane=0, window=(0, 0, 0), quorum=0):
+        """The pmo_volume, pmo_source, pmo_run and pmo_tallies of one walk, plus the device
+        launch states that must stay alive until the walk has read them."""
         if tallies.exit_state is not None and tallies.n_exit != int(n_packets):
             raise ValueError("exit_state buffer was sized for a different packet count")
         vol = self.volume_struct(skip=self.skip)
@@ -265,7 +266,13 @@ class Walker:
         for i in range(3):
             r.window[i] = int(window[i])
         r.quorum = int(quorum)
-        t = tallies.c_struct()
+        return vol, s, r, tallies.c_struct(), keep
+
+    def walk(self, src: Source, n_packets: int, seed: int, tallies: TallySet, *, hip_stream=None, **run):
+        """Launch one walk (asynchronous on torch's current stream unless given). `run` takes
+        the keywords of c_args: stream_id, first_packet, wth, roulette_m, fp64,
+        normal_fresnel, block_size, grid_blocks, packets_per_lane, window, quorum."""
+        vol, s, r, t, keep = self.c_args(src, n_packets, seed, tallies, **run)
         if hip_stream is None:
             hip_stream = torch.cuda.current_stream(self.device).cuda_stream
         with torch.cuda.device(self.device):
