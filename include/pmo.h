@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define PMO_ABI_VERSION 5
+#define PMO_ABI_VERSION 6
 
 /* ---- error codes ---------------------------------------------------------------- */
 #define PMO_OK 0
@@ -176,7 +176,9 @@ typedef struct pmo_tallies {
     const float* rings;     /* device [2*n_rings]: (r_in, r_out], mm                           */
     double* ring_w;         /* device [n_rings] or NULL                                        */
     float* exit_state;      /* device [7][n_packets] (x, y, z, vx, vy, vz, w) in the reference's
-                               get_result encoding, indexed by packet id - first_packet, or NULL */
+                               get_result encoding (z rounded to 6 decimals as np.round(.., 6)
+                               there), indexed by packet id - first_packet, or NULL; z also
+                               selects the top-face (z < 1e-8) and side-line detectors        */
     int32_t rz_nr, rz_nz;   /* internal fluence bins r in [i*rz_dr, (i+1)*rz_dr), z in
                                [k*rz_dz, (k+1)*rz_dz), r = hypot(x, y) and z of the interaction point */
     float rz_dr, rz_dz;
@@ -185,6 +187,12 @@ typedef struct pmo_tallies {
                                (pymopt/voxel_unverified/_classes.py:394-399), or NULL. Held in LDS
                                per block: rz_nr*rz_nz*8 B counts against the block's LDS (the
                                device's per-block limit, 160 KiB on gfx950)                     */
+    int32_t* exit_add;      /* device [3][n_packets] or NULL: the exit state the reference kernel
+                               returns (pymopt/voxel/_kernel.py:110-116,172), the end-label voxel
+                               each packet entered; 0 for packets ended by roulette (:134-137)   */
+    float* exit_p;          /* device [3][n_packets] or NULL: the position in that voxel's frame
+                               ([-pitch/2, pitch/2] per axis), 0 after roulette; l*(add - centre) + p
+                               is the exit point (_encooder, voxel/_classes.py:261-269)           */
 } pmo_tallies;
 
 /* ---- batched plates ----------------------------------------------------------------- */

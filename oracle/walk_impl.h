@@ -137,7 +137,8 @@ static void FN(record_exit)(const og_volume* V, const og_tallies* T, FN(ctx)* C,
     int cx = (V->nx - 1) / 2, cy = (V->ny - 1) / 2;       /* _get_center_add, voxel/_classes.py:124-128 */
     double x = l * (a[0] - cx) + (double)p[0];              /* _encooder, voxel/_classes.py:261-269 */
     double y = l * (a[1] - cy) + (double)p[1];
-    double z = l * (a[2] - 1) + (double)p[2] + l / 2;
+    /* np.round(.., 6) of the reference's z: exits through a face lie exactly on it */
+    double z = nearbyint((l * (a[2] - 1) + (double)p[2] + l / 2) * 1e6) / 1e6;
     double wd = (double)w;
     if (v[2] < 0) C->scal[0] += wd;                         /* getRdTtRate, voxel/_classes.py:285-293 */
     else if (v[2] > 0) C->scal[1] += wd;

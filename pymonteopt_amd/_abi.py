@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpmo.so")
 
-ABI_VERSION = 5          # PMO_ABI_VERSION of include/pmo.h
+ABI_VERSION = 6          # PMO_ABI_VERSION of include/pmo.h
 
 PMO_OK = 0
 PMO_ERR_INVALID = -1
@@ -68,7 +68,8 @@ class PmoTallies(C.Structure):
                 ("nl", C.c_int32), ("dl", C.c_float), ("line_y", C.c_float), ("line_z", C.c_void_p),
                 ("n_rings", C.c_int32), ("rings", C.c_void_p), ("ring_w", C.c_void_p),
                 ("exit_state", C.c_void_p), ("rz_nr", C.c_int32), ("rz_nz", C.c_int32), ("rz_dr", C.c_float),
-                ("rz_dz", C.c_float), ("fluence_rz", C.c_void_p)]
+                ("rz_dz", C.c_float), ("fluence_rz", C.c_void_p), ("exit_add", C.c_void_p),
+                ("exit_p", C.c_void_p)]
 
 
 class PmoPlateInstance(C.Structure):

@@ -22,9 +22,10 @@ __all__ = ["vmc_kernel"]
 
 
 def vmc_kernel(add, p, v, w, ma, ms, n, g, voxel_model, l, nPh, end_point, rand_seed=None):
-    """Returns (add, p, v, w) after the walk, in the layout of the reference kernel:
-    exit voxel address and in-voxel position (encoded so that l*(add - centre) + p is the
-    exit point), exit direction, exit weight; zeros for packets ended by roulette."""
+    """Returns (add, p, v, w) after the walk, as the reference kernel leaves them: the exit
+    voxel's address (the end-label voxel entered) and the position in its frame, so that
+    l*(add - centre) + p is the exit point; the exit direction and weight; zeros for packets
+    ended by roulette."""
     add = np.asarray(add)
     p = np.asarray(p, np.float32)
     v = np.asarray(v, np.float32)
@@ -53,16 +54,11 @@ def vmc_kernel(add, p, v, w, ma, ms, n, g, voxel_model, l, nPh, end_point, rand_
         src = Source(voxel=centre, w0=1.0, states=(add.astype(np.int32), p, v, w))
     seed = np.random.randint(1e5) if rand_seed is None else int(rand_seed)
     wk = Walker(vm, float(l), opt)
-    tal = wk.tallies(TallySpec(absorbed=False, exit_state=True), N)
+    tal = wk.tallies(TallySpec(absorbed=False, exit_state=True, exit_voxel=True), N)
     wk.walk(src, N, seed, tal)
-    es = tal.to_host()["exit_state"]
-    x, y, z, vx, vy, vz, ww = es
-    cx, cy = (vm.shape[0] - 1) // 2, (vm.shape[1] - 1) // 2
-    add_o = np.empty((3, N), np.int32)
-    add_o[0], add_o[1], add_o[2] = cx, cy, 1
-    p_o = np.stack([x, y, z - np.float32(l) / 2]).astype(np.float32)
-    v_o = np.stack([vx, vy, vz]).astype(np.float32)
-    killed = ww == 0
-    add_o[:, killed] = 0
-    p_o[:, killed] = 0
-    return add_o, p_o, v_o, ww.astype(np.float32)
+    r = tal.to_host()
+    # the end-label voxel each packet entered and its position there (_kernel.py:110-116),
+    # zeros for roulette kills (:134-137), as the reference kernel leaves its arrays
+    es = r["exit_state"]
+    return (r["exit_add"].astype(np.int32), r["exit_p"].astype(np.float32), es[3:6].astype(np.float32),
+            es[6].astype(np.float32))

@@ -258,6 +258,8 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     P.n_rings = tal->n_rings; P.rings = tal->rings;
     P.ring_w = tal->n_rings > 0 ? tal->ring_w : nullptr;
     P.exit_state = tal->exit_state;
+    P.exit_add = tal->exit_add;
+    P.exit_p = tal->exit_p;
     if (tal->fluence_rz) {
         if (tal->rz_nr < 1 || tal->rz_nz < 1 || !(tal->rz_dr > 0.f) || !(tal->rz_dz > 0.f))
             return fail(PMO_ERR_INVALID, "r-z fluence needs rz_nr, rz_nz >= 1 and rz_dr, rz_dz > 0");

@@ -122,6 +122,8 @@ struct WalkParams {
     const float* rings;
     double* ring_w;
     float* exit_state;
+    int32_t* exit_add;         // [3][n_packets] end voxel address, or null
+    float* exit_p;             // [3][n_packets] position in that voxel's frame, or null
     int32_t rz_nr, rz_nz;      // r-z internal fluence bins [i dr, (i+1) dr) x [k dz, (k+1) dz)
     float rz_dr, rz_dz;
     double* fluence_rz;
@@ -634,6 +636,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
                 lds_add(&bev[kBE_KILLED], 1ull);
                 lds_add(&bscal[kBS_KILL], (double)w);
                 if (P.exit_state) {
+                    // the reference zeroes add, p, v and w of a killed packet (_kernel.py:134-137);
+                    // this is the encoding of that state
                     const unsigned long long n = P.n_packets;
                     float* e = P.exit_state;
                     e[slot] = -P.pitch * (float)P.cx;
@@ -642,13 +646,24 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
                     e[3 * n + slot] = 0.f; e[4 * n + slot] = 0.f; e[5 * n + slot] = 0.f;
                     e[6 * n + slot] = 0.f;
                 }
+                if (P.exit_add) {
+                    const unsigned long long n = P.n_packets;
+                    P.exit_add[slot] = 0; P.exit_add[n + slot] = 0; P.exit_add[2 * n + slot] = 0;
+                }
+                if (P.exit_p) {
+                    const unsigned long long n = P.n_packets;
+                    P.exit_p[slot] = 0.f; P.exit_p[n + slot] = 0.f; P.exit_p[2 * n + slot] = 0.f;
+                }
             } else {
                 lds_add(&bev[kBE_ESCAPED], 1ull);
                 const double wd = (double)w;
-                // exit encoding, voxel/_classes.py:261-269
+                // exit encoding, voxel/_classes.py:261-269: z is formed in double and rounded
+                // to 6 decimals (np.round(..., 6)) before it becomes float, so exits through a
+                // face lie exactly on it (the bottom face at the plate's thickness, the top at 0)
                 const float x = P.pitch * (float)(ax - P.cx) + (float)px;
                 const float y = P.pitch * (float)(ay - P.cy) + (float)py;
-                const float z = P.pitch * (float)(az - 1) + (float)pz + 0.5f * P.pitch;
+                const double z64 = (double)P.pitch * (double)(az - 1) + (double)pz + 0.5 * (double)P.pitch;
+                const float z = (float)(__builtin_rint(z64 * 1e6) / 1e6);
                 const float fvz = (float)vz;
                 lds_add(&bscal[fvz < 0.f ? kBS_RD : (fvz > 0.f ? kBS_TT : kBS_OTHER)], wd);
                 lds_add(&bscal[kBS_FACE + 2 * xk + (xs > 0 ? 1 : 0)], wd);
@@ -688,6 +703,16 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
                     e[slot] = x; e[n + slot] = y; e[2 * n + slot] = z;
                     e[3 * n + slot] = (float)vx; e[4 * n + slot] = (float)vy; e[5 * n + slot] = fvz;
                     e[6 * n + slot] = (float)w;
+                }
+                if (P.exit_add) {
+                    // the reference kernel's exit state: the end voxel and the position in its
+                    // frame (_kernel.py:110-116)
+                    const unsigned long long n = P.n_packets;
+                    P.exit_add[slot] = ax; P.exit_add[n + slot] = ay; P.exit_add[2 * n + slot] = az;
+                }
+                if (P.exit_p) {
+                    const unsigned long long n = P.n_packets;
+                    P.exit_p[slot] = (float)px; P.exit_p[n + slot] = (float)py; P.exit_p[2 * n + slot] = (float)pz;
                 }
             }
             alive = false;

@@ -42,6 +42,7 @@ class TallySpec:
     rings: tuple = ()            # ((r_in, r_out), ...) mm, top-face annuli
     absorbed: bool = True
     exit_state: bool = False
+    exit_voxel: bool = False     # the reference kernel's exit state: end voxel + in-voxel position
     events: bool = True
     rz: tuple = ()               # (nr, dr, nz, dz): r-z internal fluence bins, mm
 
@@ -73,7 +74,11 @@ class TallySet:
             self.hist["fluence_rz"] = torch.zeros(int(spec.rz[0]) * int(spec.rz[2]), **f64)
         self.exit_state = (torch.zeros(7 * int(n_packets), dtype=torch.float32, device=device)
                            if spec.exit_state else None)
-        self.n_exit = int(n_packets) if spec.exit_state else 0
+        self.exit_add = (torch.zeros(3 * int(n_packets), dtype=torch.int32, device=device)
+                         if spec.exit_voxel else None)
+        self.exit_p = (torch.zeros(3 * int(n_packets), dtype=torch.float32, device=device)
+                       if spec.exit_voxel else None)
+        self.n_exit = int(n_packets) if (spec.exit_state or spec.exit_voxel) else 0
 
     def zero(self):
         self.scalars.zero_()
@@ -83,8 +88,9 @@ class TallySet:
             self.absorbed.zero_()
         for v in self.hist.values():
             v.zero_()
-        if self.exit_state is not None:
-            self.exit_state.zero_()
+        for t in (self.exit_state, self.exit_add, self.exit_p):
+            if t is not None:
+                t.zero_()
 
     def c_struct(self) -> _abi.PmoTallies:
         s = self.spec
@@ -102,6 +108,8 @@ class TallySet:
         t.rings = _ptr(self.rings)
         t.ring_w = _ptr(self.hist.get("ring_w"))
         t.exit_state = _ptr(self.exit_state)
+        t.exit_add = _ptr(self.exit_add)
+        t.exit_p = _ptr(self.exit_p)
         if s.rz:
             t.rz_nr, t.rz_dr, t.rz_nz, t.rz_dz = int(s.rz[0]), float(s.rz[1]), int(s.rz[2]), float(s.rz[3])
             t.fluence_rz = _ptr(self.hist["fluence_rz"])
@@ -121,6 +129,9 @@ class TallySet:
             out["fluence_rz"] = out["fluence_rz"].reshape(int(self.spec.rz[0]), int(self.spec.rz[2]))
         if self.exit_state is not None:
             out["exit_state"] = self.exit_state.cpu().numpy().reshape(7, self.n_exit)
+        if self.exit_add is not None:
+            out["exit_add"] = self.exit_add.cpu().numpy().reshape(3, self.n_exit)
+            out["exit_p"] = self.exit_p.cpu().numpy().reshape(3, self.n_exit)
         return out
 
 
@@ -236,8 +247,8 @@ class Walker:
                block_size=0, grid_blocks=0, packets_per_lane=0, window=(0, 0, 0), quorum=0):
         """The pmo_volume, pmo_source, pmo_run and pmo_tallies of one walk, plus the device
         launch states that must stay alive until the walk has read them."""
-        if tallies.exit_state is not None and tallies.n_exit != int(n_packets):
-            raise ValueError("exit_state buffer was sized for a different packet count")
+        if tallies.n_exit and tallies.n_exit != int(n_packets):
+            raise ValueError("exit-state buffers were sized for a different packet count")
         vol = self.volume_struct(skip=self.skip)
         s = _abi.PmoSource()
         s.kind = int(src.kind)

@@ -340,6 +340,7 @@ def test_compat_vmc_kernel_is_a_drop_in(slab):
     that the reference's own post-processing (_encooder, getRdTtRate) turns into the
     tallies of the same walk."""
     from pymonteopt_amd.compat import vmc_kernel
+    from pymonteopt_amd.runtime import TallySpec
     wl, wk = slab
     N = 30000
     vm = wl.labels.astype(np.int8)
@@ -364,6 +365,24 @@ def test_compat_vmc_kernel_is_a_drop_in(slab):
     r = np.sqrt(x[rd] ** 2 + y[rd] ** 2)
     h, _ = np.histogram(r, bins=np.arange(51) * 0.1, weights=w2[rd].astype(np.float64))
     assert np.allclose(h, ref["rd_r"], rtol=1e-5, atol=1e-6)
+    # the reference kernel's exit state (_kernel.py:110-116,172): add is the end-label voxel
+    # the packet entered, p its position in that voxel's frame; roulette kills are all zero
+    esc = w2 > 0
+    assert esc.sum() > 0.9 * N
+    assert np.all(vm[a2[0, esc], a2[1, esc], a2[2, esc]] == 1)
+    assert np.all(np.abs(p2[:, esc]) <= np.float32(wl.pitch / 2) * (1 + 1e-6))
+    assert np.all(a2[:, ~esc] == 0) and np.all(p2[:, ~esc] == 0)
+    # and _encooder's z (np.round(l*(add_z - 1) + p_z + l/2, 6)) is the walk's own exit z
+    spec = TallySpec(absorbed=False, exit_state=True)
+    es = gpu_run(wk, wl, N, 21, spec=spec)["exit_state"]
+    z = np.round(wl.pitch * (a2[2].astype(np.float64) - 1) + p2[2] + wl.pitch / 2, 6).astype(np.float32)
+    assert np.allclose(z[esc], es[2][esc], rtol=0, atol=2e-6)
+    assert np.allclose(x[esc], es[0][esc], rtol=0, atol=2e-6)
+    # bottom-face exits lie exactly on the plate's far face (5 mm), top-face exits at 0
+    bottom = esc & (a2[2] == vm.shape[2] - 1)
+    assert bottom.sum() > 0 and np.all(es[2][bottom] == np.float32(5.0))
+    top = esc & (a2[2] == 0)
+    assert np.all(es[2][top] == 0)
     # launch arrays that differ per packet are walked as given (PMO_BEAM_STATES): a packet's
     # history depends only on its id and its own launch, so only packet 5 changes
     p3 = p.copy()
