@@ -229,7 +229,9 @@ class PlateSweep:
     def replay(self, seed, stream_id=0):
         if self._graph is None:
             self.capture()
-        self.key.copy_(torch.tensor([int(seed) & 0x7FFFFFFF, int(stream_id) & 0x7FFFFFFF], dtype=torch.int32))
+        # the raw 32-bit key words, as run() passes them (the int32 tensor only carries the bits)
+        key = np.array([int(seed) & 0xFFFFFFFF, int(stream_id) & 0xFFFFFFFF], np.uint32).view(np.int32)
+        self.key.copy_(torch.from_numpy(key))
         self._graph.replay()
 
     def results(self):

@@ -182,6 +182,32 @@ def bone_phantom(seed=0, n=256, pitch=0.02, corr_len=0.15, porosity=0.7, cortica
     return vol
 
 
+def bone_phantom_wide(seed=0, n=256, tiles=5, pitch=0.02, corr_len=0.15, porosity=0.7, cortical=0.5, soft=1.0):
+    """The bone phantom of `bone_phantom(seed, n, ...)` widened laterally: its trabecular
+    core, a GRF drawn on a periodic grid, is tiled `tiles` x `tiles` times across x and y
+    (seamlessly), with the same soft-tissue top layer, cortical shell and depth. For odd
+    `tiles` the centre voxel falls on the same core cell as in the n^3 phantom. Returns
+    labels of shape (tiles * core + 2 * cortical + 2, same, n)."""
+    if tiles < 1 or tiles % 2 == 0:
+        raise ValueError("tiles must be odd and >= 1")
+    inner = n - 2
+    ns = int(round(soft / pitch))
+    nc = int(round(cortical / pitch))
+    if inner - 2 * nc < 1 or inner - ns - 2 * nc < 1:
+        raise ValueError("phantom too small for its cortical shell and soft-tissue layer")
+    core_shape = (inner - 2 * nc, inner - 2 * nc, inner - ns - 2 * nc)
+    f = gaussian_random_field(core_shape, pitch, corr_len, seed)
+    core = np.where(f < np.quantile(f, porosity), 1, 0).astype(np.uint8)
+    lat = tiles * core_shape[0] + 2 * nc
+    vol = np.empty((lat + 2, lat + 2, n), np.uint8)
+    interior = vol[1:-1, 1:-1, 1:-1]
+    interior[:] = 2
+    interior[:, :, :ns] = 3
+    interior[nc:lat - nc, nc:lat - nc, ns + nc:inner - nc] = np.tile(core, (tiles, tiles, 1))
+    shell(vol, 4)
+    return vol
+
+
 def bone_optics():
     """Label table of the bone phantom: bone, marrow, cortical, soft tissue; ambient 1.0."""
     return optics_table(ma=[0.02, 0.1, 0.01, 0.03], ms=[25.0, 10.0, 30.0, 12.0],

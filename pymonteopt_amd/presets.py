@@ -10,8 +10,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _abi
-from .geometry import (Source, bone_optics, bone_phantom, center_add, optics_table, plate_volume,
-                       specular)
+from .geometry import (Source, bone_optics, bone_phantom, bone_phantom_wide, center_add, optics_table,
+                       plate_volume, specular)
 
 
 @dataclass
@@ -73,6 +73,24 @@ def bone_workload(seed=0, n=256) -> Workload:
     rings = tuple((0.25 * i - 0.125, 0.25 * i + 0.125) for i in range(1, 11))
     return Workload(f"bone phantom {n}^3 @0.02mm", vol, 0.02, opt, src,
                     dict(nr=50, dr=0.05, rings=rings))
+
+
+def bone_wide_workload(seed=0, n=256, tiles=5) -> Workload:
+    """The cancellous-bone phantom of bone_workload with annular detectors at 1-10 mm
+    source-detector separation: the n^3 phantom's trabecular core (0.02 mm voxels, GRF
+    correlation length 0.15 mm, porosity 0.7) tiled 5 x 5 across the top face (21.4 mm
+    wide for n = 256), under the same 1 mm soft-tissue layer, in the same 0.5 mm cortical
+    shell; pencil beam at the top-face centre; ten 0.5 mm-wide top-face rings centred at
+    1, 2, ..., 10 mm; Rd(r) in 110 bins of 0.1 mm."""
+    vol = bone_phantom_wide(seed=seed, n=n, tiles=tiles)
+    opt = bone_optics()
+    c = center_add(vol.shape[0])
+    top = int(vol[c, c, 1])
+    w0 = float(np.float32(1.0) - np.float32(specular(1.0, float(opt[top, 3]))))
+    src = Source(voxel=(c, center_add(vol.shape[1]), 1), w0=w0)
+    rings = tuple((i - 0.25, i + 0.25) for i in range(1, 11))
+    return Workload(f"bone phantom {vol.shape[0]}x{vol.shape[1]}x{vol.shape[2]} @0.02mm (core of {n}^3 tiled "
+                    f"{tiles}x{tiles})", vol, 0.02, opt, src, dict(nr=110, dr=0.1, rings=rings))
 
 
 def grf_grid(seed=0, n=64, pitch=0.1, corr_len=0.3) -> Workload:

@@ -99,3 +99,24 @@ def test_presets():
     s = presets.skin_plate("flat")
     assert s.labels.shape == (203, 203, 142) and s.source.radius == 1.0
     assert s.source.w0 == pytest.approx(1 - (0.44 / 2.44) ** 2, rel=1e-6)
+
+
+def test_wide_bone_phantom_tiles_the_core():
+    """bone_phantom_wide: the n^3 phantom's trabecular core tiled across x and y, the same
+    layers in depth, and the centre voxel over the same core cell."""
+    kw = dict(seed=3, n=64, pitch=0.02, cortical=0.1, soft=0.2)
+    small = geometry.bone_phantom(**kw)
+    wide = geometry.bone_phantom_wide(tiles=3, **kw)
+    core = small[6:-6, 6:-6, :]                   # 1 end + 5 cortical voxels on each side
+    m = core.shape[0]
+    assert wide.shape == (3 * m + 12, 3 * m + 12, 64)
+    geometry.validate_volume(wide, 4)
+    assert np.array_equal(wide[6:-6, 6:-6, :], np.tile(core, (3, 3, 1)))
+    assert np.all(wide[1:6, 1:-1, 11:-1] == 2) and np.all(wide[1:-1, 1:-1, 1:11] == 3)
+    c_small, c_wide = geometry.center_add(64), geometry.center_add(wide.shape[0])
+    assert np.array_equal(wide[c_wide, c_wide, :], small[c_small, c_small, :])
+    with pytest.raises(ValueError):
+        geometry.bone_phantom_wide(tiles=2, **kw)
+    w = presets.bone_wide_workload(seed=0)
+    assert w.labels.shape == (1072, 1072, 256) and w.source.voxel == (535, 535, 1)
+    assert len(w.tallies["rings"]) == 10 and w.tallies["rings"][-1] == (9.75, 10.25)

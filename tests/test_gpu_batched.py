@@ -62,13 +62,17 @@ def test_graph_replay_matches_eager():
     from pymonteopt_amd.batched import PlateSweep
     sw = PlateSweep(plates(), nr=20, dr=0.1, batch=64)
     ref = {}
-    for seed in (3, 9):
-        sw.run(seed=seed)
+    big = 2 ** 31 + 5            # key words >= 2^31 reach the kernel unmasked on both paths
+    for seed in (3, 9, big):
+        sw.run(seed=seed, stream_id=big if seed == big else 0)
         torch.cuda.synchronize()
         ref[seed] = sw.results()["scalars"].copy()
+    sw.run(seed=big & 0x7FFFFFFF, stream_id=big & 0x7FFFFFFF)
+    torch.cuda.synchronize()
+    assert not np.allclose(ref[big], sw.results()["scalars"])   # the key is not masked to 31 bits
     sw.capture()
-    for seed in (3, 9, 3, 9):
-        sw.replay(seed)
+    for seed in (3, 9, big, 3, 9):
+        sw.replay(seed, stream_id=big if seed == big else 0)
         torch.cuda.synchronize()
         got = sw.results()["scalars"]
         assert np.allclose(got, ref[seed], rtol=1e-12, atol=1e-12), seed
