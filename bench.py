@@ -223,7 +223,7 @@ def adjoint_leg(n_packets=10_000_000):
         rec.update({"plain_walk_s": t_plain, "gradient_s": t_grad, "gradient_cost_vs_plain": t_grad / t_plain,
                     "plain_walk_voxel_stepping_s": t_plain_v,
                     "gradient_cost_vs_voxel_stepping_plain": t_grad / t_plain_v,
-                    "replay_mismatches": int(dw.last_mismatch.item())})
+                    "replay_mismatches": int(dw.last_mismatch)})
         out[name] = rec
     return out
 

@@ -17,6 +17,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _SO = os.path.join(_HERE, "_build", "liboracle.so")
+# ORACLE_SO names another build of the same source, e.g. the sanitizer build of `make asan`
+_LOAD = os.environ.get("ORACLE_SO") or _SO
 
 SCALAR_NAMES = ["Rd", "Tt", "other", "A", "kill", "gain", "w0",
                 "face_x-", "face_x+", "face_y-", "face_y+", "face_z-", "face_z+"]
@@ -74,9 +76,9 @@ def build(force: bool = False) -> str:
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(_SO):
+        if _LOAD == _SO and not os.path.exists(_SO):
             build()
-        L = C.CDLL(_SO)
+        L = C.CDLL(_LOAD)
         L.og_walk_f64.argtypes = [C.POINTER(OgVolume), C.POINTER(OgSource), C.POINTER(OgRun),
                                   C.POINTER(OgTallies)]
         L.og_walk_f32.argtypes = L.og_walk_f64.argtypes

@@ -90,13 +90,15 @@ class DiffWalk:
         d = PmoDiff(_p(field), self.nr, self.dr, _p(rd), _p(J), _p(rec), None, None, None)
         v, s, r = self._structs(seed, stream_id)
         with torch.cuda.device(self.device):
+            st = self.walker.launch_stream()
             rc = self.lib.pmo_walk_forward_diff(C.byref(v), C.byref(s), C.byref(r), _p(scal), None, C.byref(d),
-                                                _p(self.walker.workspace),
-                                                C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream))
-        _abi.check(rc)
+                                                _p(self.walker.workspace), C.c_void_p(st.cuda_stream))
+            _abi.check(rc)
+            self.walker.launched(st)
         norm = self.area * self.n_packets
+        # the optics table this pass walked with: the replay must use the same one
         out = {"Rd_r": rd / norm, "scalars": scal, "exit_rec": rec, "field": field, "seed": seed,
-               "stream": stream_id}
+               "stream": stream_id, "optics": self.walker.optics.clone()}
         if jac:
             mus = torch.as_tensor(self.walker.optics[: self.n_labels, 1].to(torch.float64))
             wl, wn = J[..., 0], J[..., 1]
@@ -106,7 +108,10 @@ class DiffWalk:
 
     def adjoint(self, fwd, grad_rd_r):
         """d loss / d mu_a per voxel (padded shape) for the forward pass `fwd`, given
-        d loss / d Rd(r)."""
+        d loss / d Rd(r). The replay walks with the optics table of that forward pass, even
+        if the walker's table changed since (another forward on this DiffWalk), and raises
+        if any replayed packet left differently from its recorded exit: the gradient would
+        then not be that of the forward pass."""
         g = grad_rd_r.detach().to(self.device, torch.float64).reshape(-1)
         coef = (g / (self.area * self.n_packets)).contiguous()
         grad = torch.zeros(self.shape, dtype=torch.float64, device=self.device)
@@ -114,12 +119,21 @@ class DiffWalk:
         d = PmoDiff(_p(fwd["field"]), self.nr, self.dr, None, None, _p(fwd["exit_rec"]), _p(coef), _p(grad),
                     _p(mism))
         v, s, r = self._structs(fwd["seed"], fwd["stream"])
-        with torch.cuda.device(self.device):
-            rc = self.lib.pmo_walk_adjoint(C.byref(v), C.byref(s), C.byref(r), C.byref(d),
-                                           _p(self.walker.workspace),
-                                           C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream))
-        _abi.check(rc)
-        self.last_mismatch = mism
+        current = self.walker.optics.clone()
+        self.walker.optics.copy_(fwd["optics"])              # stream-ordered before the replay
+        try:
+            with torch.cuda.device(self.device):
+                st = self.walker.launch_stream()
+                rc = self.lib.pmo_walk_adjoint(C.byref(v), C.byref(s), C.byref(r), C.byref(d),
+                                               _p(self.walker.workspace), C.c_void_p(st.cuda_stream))
+                _abi.check(rc)
+                self.walker.launched(st)
+        finally:
+            self.walker.optics.copy_(current)
+        self.last_mismatch = int(mism.item())
+        if self.last_mismatch:
+            raise RuntimeError(f"adjoint replay: {self.last_mismatch} packets left differently from the forward "
+                               "pass (volume, source, field or packet count changed since); gradient discarded")
         return grad
 
 

@@ -166,6 +166,8 @@ class Walker:
         self.optics = torch.from_numpy(optics).to(self.device)
         self.workspace = torch.zeros(int(self.lib.pmo_workspace_bytes()), dtype=torch.uint8,
                                      device=self.device)
+        self._last_launch = None        # event after the latest launch that used the workspace
+        self._inflight = []             # (event, device inputs) kept alive until the event
         if skip and not (1 <= int(skip_cap) <= _abi.PMO_SKIP_MAX_CAP):
             raise ValueError(f"skip_cap must be in [1, {_abi.PMO_SKIP_MAX_CAP}]")
         self.skip = self.nvox > SKIP_MIN_VOXELS if skip is None else bool(skip)
@@ -279,15 +281,36 @@ class Walker:
         r.quorum = int(quorum)
         return vol, s, r, tallies.c_struct(), keep
 
+    def launch_stream(self, hip_stream=None):
+        """The stream a launch that uses this Walker's workspace goes on (torch's current
+        stream unless a HIP stream handle is given), made to wait for the Walker's previous
+        launch: every launch claims packet ids from the one counter in the workspace, so
+        launches of one Walker on different streams run one after the other."""
+        st = (torch.cuda.current_stream(self.device) if hip_stream is None
+              else torch.cuda.ExternalStream(hip_stream, device=self.device))
+        if self._last_launch is not None:
+            st.wait_event(self._last_launch)
+        return st
+
+    def launched(self, st, keep=()):
+        """Record a launch made on `st`; `keep` (device inputs the kernel reads) stays alive
+        until that launch has completed."""
+        ev = torch.cuda.Event()
+        ev.record(st)
+        self._last_launch = ev
+        self._inflight = [(e, k) for e, k in self._inflight if not e.query()]
+        if keep:
+            self._inflight.append((ev, keep))
+
     def walk(self, src: Source, n_packets: int, seed: int, tallies: TallySet, *, hip_stream=None, **run):
-        """Launch one walk (asynchronous on torch's current stream unless given). `run` takes
-        the keywords of c_args: stream_id, first_packet, wth, roulette_m, fp64,
-        normal_fresnel, block_size, grid_blocks, packets_per_lane, window, quorum."""
+        """Launch one walk (asynchronous on torch's current stream unless a HIP stream handle
+        is given; after this Walker's previous launch, whatever its stream). `run` takes the
+        keywords of c_args: stream_id, first_packet, wth, roulette_m, fp64, normal_fresnel,
+        block_size, grid_blocks, packets_per_lane, window, quorum."""
         vol, s, r, t, keep = self.c_args(src, n_packets, seed, tallies, **run)
-        if hip_stream is None:
-            hip_stream = torch.cuda.current_stream(self.device).cuda_stream
         with torch.cuda.device(self.device):
+            st = self.launch_stream(hip_stream)
             rc = self.lib.pmo_walk(C.byref(vol), C.byref(s), C.byref(r), C.byref(t),
-                                   _ptr(self.workspace), C.c_void_p(hip_stream))
-        _abi.check(rc)
-        self._keep_states = keep          # alive until the asynchronous walk has read them
+                                   _ptr(self.workspace), C.c_void_p(st.cuda_stream))
+            _abi.check(rc)
+            self.launched(st, keep)

@@ -91,7 +91,7 @@ def test_voxel_gradient_by_replay(grid):
     fwd = dg.forward(seed=5, mua_field=field, jac=True)
     grad = dg.adjoint(fwd, gb)
     torch.cuda.synchronize()
-    assert int(dg.last_mismatch.item()) == 0     # every replayed path ended as recorded
+    assert dg.last_mismatch == 0     # every replayed path ended as recorded
     gc = grad.cpu().numpy()
     Jl = (fwd["J_mua"] * gb[:, None]).sum(0).cpu().numpy()
     for j in range(grid.n_labels):
@@ -116,6 +116,22 @@ def test_voxel_gradient_by_replay(grid):
     fld = field.clone().double().requires_grad_(True)
     (rd_of_absorption_field(fld, dg, seed=5) * gb).sum().backward()
     assert torch.allclose(fld.grad, grad, rtol=1e-9, atol=1e-15)
+    # a per-label forward on the same DiffWalk between the field's forward and its backward
+    # rewrites the walker's optics table; the replay still walks with the forward's table
+    from pymonteopt_amd.adjoint import rd_of_label_optics
+    fld2 = field.clone().double().requires_grad_(True)
+    y = (rd_of_absorption_field(fld2, dg, seed=5) * gb).sum()
+    L = grid.n_labels
+    mua = torch.tensor(grid.optics[:L, 0] * 1.3, dtype=torch.float64, device=dg.device)
+    mus = torch.tensor(grid.optics[:L, 1] * 0.7, dtype=torch.float64, device=dg.device)
+    rd_of_label_optics(mua, mus, dg, seed=9)
+    y.backward()
+    assert dg.last_mismatch == 0
+    assert torch.allclose(fld2.grad, grad, rtol=1e-9, atol=1e-15)
+    # a record replayed under another key diverges: the adjoint refuses the gradient
+    with pytest.raises(RuntimeError, match="left differently"):
+        dg.adjoint(dict(fwd, seed=6), gb)
+    assert dg.last_mismatch > 0
 
 
 def test_adjoint_cost_vs_forward(grid, skin):

@@ -481,6 +481,29 @@ def test_compat_takes_per_packet_launch_arrays(slab):
     assert w_o[v_o[2] > 0].sum(dtype=np.float64) == pytest.approx(r["scalars"]["Tt"], rel=1e-6)
 
 
+def test_walks_of_one_walker_on_two_streams(slab):
+    """Two walks of one Walker queued on two streams share its claim counter; the second
+    waits for the first, so each gets all of its packets and the tallies of a walk run alone."""
+    import torch
+    from pymonteopt_amd.runtime import TallySpec
+    wl, wk = slab
+    N = 200000
+    spec = TallySpec(nr=50, dr=0.1)
+    ref = [gpu_run(wk, wl, N, s, spec=spec) for s in (41, 42)]
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    t1, t2 = wk.tallies(spec), wk.tallies(spec)
+    torch.cuda.synchronize()
+    wk.walk(wl.source, N, 41, t1, hip_stream=s1.cuda_stream)
+    wk.walk(wl.source, N, 42, t2, hip_stream=s2.cuda_stream)
+    torch.cuda.synchronize()
+    for t, r in ((t1, ref[0]), (t2, ref[1])):
+        got = t.to_host(shape=wk.shape)
+        assert got["events"]["packets"] == N
+        for k in ("packets", "interactions", "crossings", "fresnel", "escaped", "killed"):
+            assert got["events"][k] == r["events"][k], k
+        assert got["scalars"]["Rd"] == pytest.approx(r["scalars"]["Rd"], rel=1e-12)
+
+
 def test_packet_range_shards_sum_to_the_whole_run(slab):
     """The multi-GPU design gives each GPU a range of packet ids under one Philox key. On one
     GPU: four launches over [k N/4, (k+1) N/4) (first_packet) into one set of tallies give
