@@ -289,15 +289,17 @@ def turing_leg():
             "cell_steps_per_s": n ** 3 * steps / t.time_s, "bv_tv_real": bv, "peaks": peaks}
 
 
-def bone_leg(peaks, n_packets=100_000_000):
-    """The 256^3 cancellous-bone phantom at 0.02 mm (GRF trabecular core, cortical shell,
-    soft-tissue top): one launch of 1e8 packets with the absorbed map, Rd(r) and the ten
-    annular top-face detectors, after a 1e6-packet warm-up. Fresnel at every bone/marrow
-    face makes this the crossing- and Fresnel-heavy case."""
+def bone_leg(peaks, n_packets=100_000_000, wide=True):
+    """The cancellous-bone phantom at 0.02 mm (GRF trabecular core, cortical shell,
+    soft-tissue top): one launch of 1e8 packets with the absorbed map, Rd(r) and ten annular
+    top-face detectors, after a 1e6-packet warm-up. Fresnel at every bone/marrow face makes
+    this the crossing- and Fresnel-heavy case. wide: the 256^3 phantom's core tiled 5 x 5
+    (1072 x 1072 x 256 voxels, 21.4 mm wide) with the rings centred at 1..10 mm; else the
+    256^3 phantom itself with rings at 0.25..2.5 mm."""
     import torch
     from pymonteopt_amd import _abi, presets, roofline
     from pymonteopt_amd.runtime import TallySpec, Walker
-    wl = presets.bone_workload(seed=0)
+    wl = presets.bone_wide_workload(seed=0) if wide else presets.bone_workload(seed=0)
     wk = Walker(wl.labels, wl.pitch, wl.optics)
     tal = wk.tallies(TallySpec(**wl.tallies))
     wk.walk(wl.source, 1_000_000, 99, tal)
@@ -313,8 +315,9 @@ def bone_leg(peaks, n_packets=100_000_000):
     ev = dict(zip(_abi.EVENT_NAMES, [int(x) for x in tal.events.cpu().tolist()]))
     ops = roofline.ops_per_packet(ev)
     glob_pp = ev["global_deposits"] / n_packets
-    rings = tal.to_host()["ring_w"] / n_packets
+    rings = tal.hist["ring_w"].cpu().numpy() / n_packets
     return {"workload": wl.name + f", {n_packets:.0e} packets", "seconds": t, "photons_per_s": n_packets / t,
+            "rings_mm": [list(r) for r in wl.tallies["rings"]],
             "events_per_packet": {k: ev[k] / n_packets for k in ("interactions", "crossings", "fresnel", "roulette")},
             "Rd": s["Rd"] / n_packets, "Tt": s["Tt"] / n_packets, "A": s["A"] / n_packets,
             "ring_weight_per_packet": [float(x) for x in rings],
@@ -681,7 +684,8 @@ def main():
             out["obd"], obd_sample = obd_leg()
             out["turing"] = turing_leg()
             out["virtual_obd"] = virtual_obd_leg()
-            out["bone"] = bone_leg(peaks)
+            out["bone"] = bone_leg(peaks, wide=True)
+            out["bone_256"] = bone_leg(peaks, wide=False)
             out["published_slab"] = published_slab_leg()
         if not args.no_cpu:
             out["cpu_baseline"] = cpu_baseline(wl)

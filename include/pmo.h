@@ -308,14 +308,18 @@ int pmo_walk_adjoint(const pmo_volume* vol, const pmo_source* src, const pmo_run
  * pmo_probe_atomics issues blocks*256*iters device-scope atomic adds (no return) of 1 to
  * pseudo-random addresses of the device array buf[n], whose element type `kind` names; with
  * PMO_PROBE_F64_PER_XCD, buf holds 8 copies of n doubles and each workgroup adds into the
- * copy of the XCD it runs on. pmo_probe_valu runs blocks*256 lanes that each issue
- * iters x 32 independent FP32 FMAs and write one float to out[blocks*256]. */
+ * copy of the XCD it runs on (PMO_PROBE_F64_PER_XCD_WG: the same at workgroup scope);
+ * with PMO_PROBE_F64_LINES each wave adds to 64 consecutive doubles (eight 64 B lines) of a
+ * pseudo-random 512 B block of buf[n], n >= 64. pmo_probe_valu runs blocks*256 lanes that
+ * each issue iters x 32 independent FP32 FMAs and write one float to out[blocks*256]. */
 enum {
     PMO_PROBE_F64 = 0,
     PMO_PROBE_F32 = 1,
     PMO_PROBE_U64 = 2,
     PMO_PROBE_U32 = 3,
-    PMO_PROBE_F64_PER_XCD = 4
+    PMO_PROBE_F64_PER_XCD = 4,
+    PMO_PROBE_F64_PER_XCD_WG = 5,
+    PMO_PROBE_F64_LINES = 6
 };
 int pmo_probe_atomics(void* buf, int64_t n, int32_t blocks, int32_t iters, int32_t kind, void* stream);
 int pmo_probe_valu(float* out, int32_t blocks, int32_t iters, void* stream);

@@ -31,6 +31,7 @@ PMO_F_LAYER_STEP = 1 << 2
 PMO_SKIP_MAX_CAP = 127
 
 PMO_PROBE_F64, PMO_PROBE_F32, PMO_PROBE_U64, PMO_PROBE_U32, PMO_PROBE_F64_PER_XCD = 0, 1, 2, 3, 4
+PMO_PROBE_F64_PER_XCD_WG, PMO_PROBE_F64_LINES = 5, 6
 
 SCALAR_NAMES = ["Rd", "Tt", "other", "A", "kill", "gain", "w0",
                 "face_x-", "face_x+", "face_y-", "face_y+", "face_z-", "face_z+"]

@@ -463,6 +463,14 @@ int pmo_probe_atomics(void* buf, int64_t n, int32_t blocks, int32_t iters, int32
     case PMO_PROBE_F64_PER_XCD:
         hipLaunchKernelGGL(pmo::probe_atomics_kernel<double>, dim3(blocks), dim3(256), 0, s, (double*)buf, m, iters, 1);
         break;
+    case PMO_PROBE_F64_PER_XCD_WG:
+        hipLaunchKernelGGL((pmo::probe_atomics_kernel<double, true>), dim3(blocks), dim3(256), 0, s, (double*)buf, m,
+                           iters, 1);
+        break;
+    case PMO_PROBE_F64_LINES:
+        if (m < 64) return fail(PMO_ERR_INVALID, "the line probe needs n >= 64");
+        hipLaunchKernelGGL(pmo::probe_atomics_lines_kernel, dim3(blocks), dim3(256), 0, s, (double*)buf, m, iters);
+        break;
     default:
         return fail(PMO_ERR_INVALID, "unknown probe kind");
     }

@@ -792,16 +792,32 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n)
 // XCD (accelerator complex die) the calling wave runs on: HW_REG_XCC_ID (hwreg 20), bits 3:0
 __device__ __forceinline__ uint32_t xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 7u; }
 
-// Device-scope atomic adds (no return) of T to pseudo-random addresses of buf[n]; with
-// per_xcd, buf holds 8 copies of n elements and each workgroup adds into its XCD's copy.
-template <class T>
+// Atomic adds (no return) of T to pseudo-random addresses of buf[n], at device scope or
+// (WG) workgroup scope; with per_xcd, buf holds 8 copies of n elements and each workgroup
+// adds into its XCD's copy. On gfx950 both scopes compile to the same global_atomic_add
+// (no scope bits), so they probe the same path.
+template <class T, bool WG = false>
 __global__ void probe_atomics_kernel(T* buf, uint32_t n, int iters, int per_xcd)
 {
     uint32_t x = (blockIdx.x * blockDim.x + threadIdx.x) * 2654435761u ^ 2463534242u;
     T* base = per_xcd ? buf + (size_t)xcc_id() * n : buf;
     for (int i = 0; i < iters; ++i) {
         x ^= x << 13; x ^= x >> 17; x ^= x << 5;          // xorshift32
-        __hip_atomic_fetch_add(base + (x % n), (T)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (WG) __hip_atomic_fetch_add(base + (x % n), (T)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else __hip_atomic_fetch_add(base + (x % n), (T)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// The same adds with each wave's 64 lanes on 64 consecutive doubles (eight 64 B lines) of a
+// pseudo-random 512 B block of buf[n] (n >= 64): the rate per element when every request
+// carries eight adds to one line.
+__global__ void probe_atomics_lines_kernel(double* buf, uint32_t n, int iters)
+{
+    uint32_t x = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 2654435761u ^ 2463534242u;
+    const uint32_t lane = threadIdx.x & 63u, nb = n >> 6;
+    for (int i = 0; i < iters; ++i) {
+        x ^= x << 13; x ^= x >> 17; x ^= x << 5;
+        __hip_atomic_fetch_add(buf + (size_t)(x % nb) * 64u + lane, 1.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 

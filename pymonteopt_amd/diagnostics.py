@@ -55,15 +55,24 @@ def peaks():
                                                              kind, s)))
             out[f"{name}_atomics_per_s_{mib}MiB"] = count / t
         if mib in (2, 32):
-            # 8 copies of `mib` MiB, each workgroup adding into the copy of its own XCD
+            # 8 copies of `mib` MiB, each workgroup adding into the copy of its own XCD, at
+            # device and at workgroup scope
             n = int(mib * 1024 * 1024 // 8)
-            buf = torch.zeros(8 * n, dtype=torch.float64, device="cuda:0")
+            for tag, kind in (("", _abi.PMO_PROBE_F64_PER_XCD), ("_workgroup_scope", _abi.PMO_PROBE_F64_PER_XCD_WG)):
+                buf = torch.zeros(8 * n, dtype=torch.float64, device="cuda:0")
+                t = timed(lambda: _abi.check(L.pmo_probe_atomics(C.c_void_p(buf.data_ptr()), n, blocks, iters,
+                                                                 kind, s)))
+                out[f"f64_per_xcd_copy{tag}_atomics_per_s_{mib}MiB"] = count / t
+                torch.cuda.synchronize()
+                # every add of 1.0 must have landed: a lost update would show as a short sum
+                out[f"f64_per_xcd_copy{tag}_sum_ok_{mib}MiB"] = bool(float(buf.sum()) == 2.0 * count)
+            # each wave's adds on eight consecutive 64 B lines
+            buf = torch.zeros(n, dtype=torch.float64, device="cuda:0")
             t = timed(lambda: _abi.check(L.pmo_probe_atomics(C.c_void_p(buf.data_ptr()), n, blocks, iters,
-                                                             _abi.PMO_PROBE_F64_PER_XCD, s)))
-            out[f"f64_per_xcd_copy_atomics_per_s_{mib}MiB"] = count / t
+                                                             _abi.PMO_PROBE_F64_LINES, s)))
+            out[f"f64_wave_on_8_lines_atomics_per_s_{mib}MiB"] = count / t
             torch.cuda.synchronize()
-            # every add of 1.0 must have landed: a lost update would show as a short sum
-            out[f"f64_per_xcd_copy_sum_ok_{mib}MiB"] = bool(float(buf.sum()) == 2.0 * count)
+            out[f"f64_wave_on_8_lines_sum_ok_{mib}MiB"] = bool(float(buf.sum()) == 2.0 * count)
     o = torch.empty(2048 * 256, dtype=torch.float32, device="cuda:0")
     t = timed(lambda: _abi.check(L.pmo_probe_valu(C.c_void_p(o.data_ptr()), 2048, 8192, s)))
     out["valu_fp32_fma_lane_ops_per_s"] = 2048 * 256.0 * 8192 * 32 / t
