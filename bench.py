@@ -301,13 +301,15 @@ def bone_leg(peaks, n_packets=100_000_000, wide=True):
     from pymonteopt_amd.runtime import TallySpec, Walker
     wl = presets.bone_wide_workload(seed=0) if wide else presets.bone_workload(seed=0)
     wk = Walker(wl.labels, wl.pitch, wl.optics)
-    tal = wk.tallies(TallySpec(**wl.tallies))
-    wk.walk(wl.source, 1_000_000, 99, tal)
+    spec = TallySpec(**wl.tallies)
+    kw, _ = wk.tune(wl.source, spec, n_packets=5_000_000)
+    tal = wk.tallies(spec)
+    wk.walk(wl.source, 1_000_000, 99, tal, **kw)
     tal.zero()
     torch.cuda.synchronize()
     e0, e1 = _events()
     e0.record()
-    wk.walk(wl.source, n_packets, 5, tal)
+    wk.walk(wl.source, n_packets, 5, tal, **kw)
     e1.record()
     e1.synchronize()
     t = e0.elapsed_time(e1) * 1e-3
@@ -317,7 +319,7 @@ def bone_leg(peaks, n_packets=100_000_000, wide=True):
     glob_pp = ev["global_deposits"] / n_packets
     rings = tal.hist["ring_w"].cpu().numpy() / n_packets
     return {"workload": wl.name + f", {n_packets:.0e} packets", "seconds": t, "photons_per_s": n_packets / t,
-            "rings_mm": [list(r) for r in wl.tallies["rings"]],
+            "launch": kw or "library default", "rings_mm": [list(r) for r in wl.tallies["rings"]],
             "events_per_packet": {k: ev[k] / n_packets for k in ("interactions", "crossings", "fresnel", "roulette")},
             "Rd": s["Rd"] / n_packets, "Tt": s["Tt"] / n_packets, "A": s["A"] / n_packets,
             "ring_weight_per_packet": [float(x) for x in rings],
@@ -540,6 +542,7 @@ def main():
     ap.add_argument("--window", default="0,0,0", help="LDS tally window x,y,z (0 = default, -1 = none)")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched and adjoint legs")
     ap.add_argument("--quorum", type=int, default=0, help="lanes ready before a wave runs interactions")
+    ap.add_argument("--no-tune", action="store_true", help="take the library's default launch shape")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported. The walk would shard "
@@ -562,6 +565,13 @@ def main():
     N = int(args.packets)
     kw = dict(block_size=args.block, packets_per_lane=args.ppl, fp64=args.fp64,
               window=tuple(int(x) for x in args.window.split(",")), quorum=args.quorum)
+    launch = "library default"
+    if not args.no_tune and args.block == 0 and args.window == "0,0,0" and args.quorum == 0 and not args.fp64:
+        # the launch shape (block size, LDS window, quorum) picked by timing a 1e7-packet walk
+        # of each candidate, before the timed region
+        tuned, _ = wk.tune(wl.source, spec, n_packets=min(N, 10_000_000))
+        kw.update(tuned)
+        launch = tuned or "library default"
 
     for i in range(args.warmup):
         tal.zero()
@@ -626,6 +636,7 @@ def main():
         "config": {
             "workload": wl.name + f", {N:.0e} packets per step",
             "packets_per_step": N,
+            "launch": launch,
             "tallies": "Rd, Tt, Rd(r) 40x0.1mm, absorbed/fluence map (FP64)" if spec.absorbed
             else "Rd, Tt, Rd(r) 40x0.1mm (FP64)",
             "walk_kernel_ms_mean": 1e3 * mean_launch_s,

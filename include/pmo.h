@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define PMO_ABI_VERSION 6
+#define PMO_ABI_VERSION 7
 
 /* ---- error codes ---------------------------------------------------------------- */
 #define PMO_OK 0
@@ -91,7 +91,9 @@ enum {
     PMO_EV_ROULETTE = 4, /* roulette tests (w <= wth)                                         */
     PMO_EV_KILLED = 5,   /* packets ended by roulette                                         */
     PMO_EV_ESCAPED = 6,  /* packets that entered an end (ambient) voxel                       */
-    PMO_EV_GLOBAL_DEPOSITS = 7, /* FP64 global atomics issued for the absorbed-weight map        */
+    PMO_EV_GLOBAL_DEPOSITS = 7, /* global atomic requests for the absorbed-weight map: one per
+                                   voxel deposit outside the LDS window, or with absorbed_lines
+                                   one per 64 B cube line handed over                        */
     PMO_EV_COUNT = 8
 };
 
@@ -193,7 +195,19 @@ typedef struct pmo_tallies {
     float* exit_p;          /* device [3][n_packets] or NULL: the position in that voxel's frame
                                ([-pitch/2, pitch/2] per axis), 0 after roulette; l*(add - centre) + p
                                is the exit point (_encooder, voxel/_classes.py:261-269)           */
+    double* absorbed_lines; /* device scratch of PMO_ABSORBED_LINES_DOUBLES(nx, ny, nz) doubles, or
+                               NULL. With it pmo_walk gathers the absorbed weight outside its LDS
+                               window per 2x2x2-voxel cube (one 64 B line of FP64) in per-lane LDS
+                               buffers and adds each cube's line with one atomic request, then adds
+                               the scratch into `absorbed`; the library clears it. NULL: one FP64
+                               atomic per voxel deposit. Either way `absorbed` gets the same sums
+                               (up to FP64 summation order).                                    */
 } pmo_tallies;
+
+/* doubles of pmo_tallies.absorbed_lines for an nx x ny x nz volume: 8 per 2x2x2 cube */
+#define PMO_ABSORBED_LINES_DOUBLES(nx, ny, nz) \
+    (8ull * (unsigned long long)(((nx) + 1) / 2) * (unsigned long long)(((ny) + 1) / 2) * \
+     (unsigned long long)(((nz) + 1) / 2))
 
 /* ---- batched plates ----------------------------------------------------------------- */
 /* One instance of a batched sweep: a plate (pymopt/voxel/_classes.py:352-424 PlateModel

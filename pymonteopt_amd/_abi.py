@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpmo.so")
 
-ABI_VERSION = 6          # PMO_ABI_VERSION of include/pmo.h
+ABI_VERSION = 7          # PMO_ABI_VERSION of include/pmo.h
 
 PMO_OK = 0
 PMO_ERR_INVALID = -1
@@ -70,7 +70,12 @@ class PmoTallies(C.Structure):
                 ("n_rings", C.c_int32), ("rings", C.c_void_p), ("ring_w", C.c_void_p),
                 ("exit_state", C.c_void_p), ("rz_nr", C.c_int32), ("rz_nz", C.c_int32), ("rz_dr", C.c_float),
                 ("rz_dz", C.c_float), ("fluence_rz", C.c_void_p), ("exit_add", C.c_void_p),
-                ("exit_p", C.c_void_p)]
+                ("exit_p", C.c_void_p), ("absorbed_lines", C.c_void_p)]
+
+
+def absorbed_lines_doubles(nx, ny, nz):
+    """PMO_ABSORBED_LINES_DOUBLES: the absorbed_lines scratch of an nx x ny x nz volume."""
+    return 8 * ((nx + 1) // 2) * ((ny + 1) // 2) * ((nz + 1) // 2)
 
 
 class PmoPlateInstance(C.Structure):

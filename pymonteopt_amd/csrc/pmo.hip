@@ -58,10 +58,21 @@ constexpr unsigned long long kWholeVolumeCells = (unsigned long long)kDefaultWin
 // on the 256^3 bone phantom with either launch shape, +2% on the skin plate without the
 // absorbed map, a tie with it, -2% on the slab
 constexpr int kDefaultQuorum = 32;
+// With the absorbed map through cube lines (absorbed_lines), a quorum of 40 and windows of at
+// most 22x22x22 (wide launch: one 1024-lane block per CU, the window beside 64 KiB of line
+// buffers) or 12x12x12 (256-lane blocks, as many per CU as the registers allow, 5, each
+// window in its block's share of the LDS beside 16 KiB of line buffers). On the MI355X at 1e8
+// packets: skin plate 1.56e8 photons/s wide (1.43e8 with 256-lane blocks; 1.28e8 with
+// per-voxel atomics); 256^3 bone phantom 1.44e8 wide, 1.56e8 with 256-lane blocks and a
+// 9x9x21 window (1.40e8 per voxel).
+constexpr int kLinesWideWindow[3] = {22, 22, 22};
+constexpr int kLinesWindow[3] = {12, 12, 12};
+constexpr int kLinesQuorum = 40;
 
 struct DevInfo {
     int n_cu = 0;
-    size_t max_lds = kMaxLds;  // LDS one block may allocate (160 KiB on gfx950)
+    size_t max_lds = kMaxLds;     // LDS one block may allocate (160 KiB on gfx950)
+    size_t lds_per_cu = kMaxLds;  // LDS of one CU, shared by its resident blocks
     bool ok = false;
 };
 
@@ -84,6 +95,10 @@ int device_info(DevInfo* out)
         e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
         if (e != hipSuccess) return hip_fail(e, "hipDeviceGetAttribute");
         if (lds > 0) g_dev[dev].max_lds = (size_t)lds;
+        int lcu = 0;
+        e = hipDeviceGetAttribute(&lcu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
+        if (e != hipSuccess) return hip_fail(e, "hipDeviceGetAttribute");
+        g_dev[dev].lds_per_cu = lcu > 0 ? (size_t)lcu : g_dev[dev].max_lds;
         g_dev[dev].ok = true;
     }
     *out = g_dev[dev];
@@ -91,11 +106,14 @@ int device_info(DevInfo* out)
 }
 
 template <typename R>
-int blocks_per_cu(int block, size_t lds, int* out, bool rz = false, bool skip = false)
+int blocks_per_cu(int block, size_t lds, int* out, bool rz = false, bool skip = false, bool lines = false)
 {
     int nb = 0;
     hipError_t e;
-    if (skip)
+    if (lines)
+        e = skip ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::walk_lines_kernel<true>, block, lds)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::walk_lines_kernel<false>, block, lds);
+    else if (skip)
         e = rz ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::walk_kernel<R, true, true>, block, lds)
                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::walk_kernel<R, false, true>, block, lds);
     else
@@ -201,6 +219,12 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     if (!(run->roulette_m >= 1.f)) return fail(PMO_ERR_INVALID, "roulette_m must be >= 1");
     if (run->n_packets == 0) return PMO_OK;
     const bool whole = nvox <= kWholeVolumeCells;
+    const bool fp64 = (run->flags & PMO_F_FP64) != 0u;
+    // the absorbed map through cube lines (the FP32 walk only; the FP64 build and the r-z
+    // fluence variant deposit voxel by voxel); a volume held whole in the window needs none
+    const bool win_default = run->window[0] == 0 && run->window[1] == 0 && run->window[2] == 0;
+    const bool lines = tal->absorbed != nullptr && tal->absorbed_lines != nullptr && !fp64 && !tal->fluence_rz &&
+                       !(whole && win_default);
     const bool wide = run->n_packets >= kWideRunPackets && tal->absorbed != nullptr && !whole;
     const int bs = run->block_size > 0 ? run->block_size : (wide ? kWideBlock : kDefaultBlock);
     if (bs % 64 != 0 || bs > 1024) return fail(PMO_ERR_INVALID, "block_size %d must be a multiple of 64, <= 1024", bs);
@@ -210,7 +234,6 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     if (tal->nr > 0 && !(tal->dr > 0.f)) return fail(PMO_ERR_INVALID, "dr must be > 0");
     if (tal->nl > 0 && !(tal->dl > 0.f)) return fail(PMO_ERR_INVALID, "dl must be > 0");
     if (tal->n_rings > 0 && !tal->rings) return fail(PMO_ERR_INVALID, "rings pointer is null");
-    const bool fp64 = (run->flags & PMO_F_FP64) != 0u;
 
     DevInfo d;
     int rc = device_info(&d);
@@ -242,7 +265,7 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     P.max_events = kMaxEvents;
     if (run->quorum < 0 || run->quorum > 64) return fail(PMO_ERR_INVALID, "quorum %d outside [0, 64]", run->quorum);
     // >= 1, so a wave whose lanes all wait for crossings still advances them
-    P.quorum = run->quorum > 0 ? (uint32_t)run->quorum : (uint32_t)kDefaultQuorum;
+    P.quorum = run->quorum > 0 ? (uint32_t)run->quorum : (uint32_t)(lines ? kLinesQuorum : kDefaultQuorum);
     P.scalars = tal->scalars;
     P.events = reinterpret_cast<unsigned long long*>(tal->events);
     P.absorbed = tal->absorbed;
@@ -288,15 +311,23 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     P.h_ring = off; off += P.ring_w ? (uint32_t)P.n_rings : 0u;
     P.h_rz = off; off += P.fluence_rz ? (uint32_t)(P.rz_nr * P.rz_nz) : 0u;
     P.h_total = off;
-    P.lds_win = (uint32_t)align_up(P.lds_hist + (size_t)P.h_total * sizeof(double), 16);
+    // line buffers (8 doubles per lane) and their hand-over stage (16 words per wave)
+    P.lines = lines ? tal->absorbed_lines : nullptr;
+    P.lbuf_stride = (uint32_t)bs;
+    P.nly = (uint32_t)(vol->ny + 1) / 2u;
+    P.nlz = (uint32_t)(vol->nz + 1) / 2u;
+    P.lds_lbuf = (uint32_t)align_up(P.lds_hist + (size_t)P.h_total * sizeof(double), 16);
+    P.lds_stage = P.lds_lbuf + (lines ? (uint32_t)(8u * (uint32_t)bs * sizeof(double)) : 0u);
+    P.lds_win = P.lds_stage + (lines ? (uint32_t)((uint32_t)bs / 64u * 16u * sizeof(int32_t)) : 0u);
     if (P.lds_win > d.max_lds)
-        return fail(PMO_ERR_UNSUPPORTED, "histograms need %u B of LDS (limit %zu)", P.lds_win, d.max_lds);
+        return fail(PMO_ERR_UNSUPPORTED, "histograms and line buffers need %u B of LDS (limit %zu)", P.lds_win,
+                    d.max_lds);
 
     // LDS window of the absorbed tally: the launch voxel region, where deposits collide. A
     // default window shrinks (depth first) to the LDS the histograms leave; one the caller
     // sizes must fit as given.
-    const bool win_default = run->window[0] == 0 && run->window[1] == 0 && run->window[2] == 0;
-    const int* wdef = bs >= kWideBlock ? kWideWindow : kDefaultWindow;
+    const int* wdef = lines ? (bs >= kWideBlock ? kLinesWideWindow : kLinesWindow)
+                            : (bs >= kWideBlock ? kWideWindow : kDefaultWindow);
     int wdim[3];
     const int ext[3] = {vol->nx, vol->ny, vol->nz};
     for (int a = 0; a < 3; ++a) {
@@ -305,7 +336,17 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     }
     for (int a = 0; a < 3; ++a)
         if (wdim[a] > ext[a]) wdim[a] = ext[a];
-    const size_t win_budget = (d.max_lds - P.lds_win) / sizeof(double);
+    size_t win_budget = (d.max_lds - P.lds_win) / sizeof(double);
+    if (lines && win_default) {
+        // keep as many blocks per CU as the registers allow: each gets its share of the CU's
+        // LDS, the line buffers first, the window from the rest
+        int nb_reg = 0;
+        rc = blocks_per_cu<float>(bs, 0, &nb_reg, false, P.skip != nullptr, true);
+        if (rc != PMO_OK) return rc;
+        const size_t share = d.lds_per_cu / (size_t)(nb_reg > 0 ? nb_reg : 1);
+        const size_t cells = share > P.lds_win ? (share - P.lds_win) / sizeof(double) : 0;
+        if (cells < win_budget) win_budget = cells;
+    }
     if (win_default)
         while ((size_t)wdim[0] * wdim[1] * wdim[2] > win_budget) {
             if (wdim[2] > wdim[0] / 2 && wdim[2] > 1) --wdim[2];
@@ -334,7 +375,7 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     int nb = 0;
     const bool skip = P.skip != nullptr;
     rc = fp64 ? blocks_per_cu<double>(bs, lds, &nb, P.fluence_rz != nullptr, skip)
-              : blocks_per_cu<float>(bs, lds, &nb, P.fluence_rz != nullptr, skip);
+              : blocks_per_cu<float>(bs, lds, &nb, P.fluence_rz != nullptr, skip, lines);
     if (rc != PMO_OK) return rc;
     unsigned long long grid = run->grid_blocks > 0 ? (unsigned long long)run->grid_blocks
                                                    : (unsigned long long)nb * (unsigned long long)d.n_cu;
@@ -357,6 +398,14 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     hipLaunchKernelGGL(pmo::zero_u64_kernel, dim3(1), dim3(64), 0, s, P.work, 4);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "zero workspace launch");
+    const unsigned long long n_cubes = (unsigned long long)((vol->nx + 1) / 2) * P.nly * P.nlz;
+    const unsigned fill_grid = (unsigned)((n_cubes * 8ull + 255ull) / 256ull < 8192ull ? (n_cubes * 8ull + 255ull) / 256ull
+                                                                                        : 8192ull);
+    if (lines) {
+        hipLaunchKernelGGL(pmo::zero_f64_kernel, dim3(fill_grid), dim3(256), 0, s, P.lines, n_cubes * 8ull);
+        e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "zero line scratch launch");
+    }
     const dim3 gd((unsigned)grid), bd(bs);
     const bool rz = P.fluence_rz != nullptr;
     if (fp64) {
@@ -367,6 +416,9 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
             if (rz) hipLaunchKernelGGL((pmo::walk_kernel<double, true, false>), gd, bd, lds, s, P);
             else hipLaunchKernelGGL((pmo::walk_kernel<double, false, false>), gd, bd, lds, s, P);
         }
+    } else if (lines) {
+        if (skip) hipLaunchKernelGGL(pmo::walk_lines_kernel<true>, gd, bd, lds, s, P);
+        else hipLaunchKernelGGL(pmo::walk_lines_kernel<false>, gd, bd, lds, s, P);
     } else {
         if (skip) {
             if (rz) hipLaunchKernelGGL((pmo::walk_kernel<float, true, true>), gd, bd, lds, s, P);
@@ -378,6 +430,13 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     }
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "walk launch");
+    if (lines) {
+        const unsigned g = (unsigned)((n_cubes + 255ull) / 256ull < 8192ull ? (n_cubes + 255ull) / 256ull : 8192ull);
+        hipLaunchKernelGGL(pmo::lines_to_absorbed_kernel, dim3(g), dim3(256), 0, s, (const double*)P.lines, P.absorbed,
+                           P.nx, P.ny, P.nz, P.nly, P.nlz, n_cubes);
+        e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "line scratch transfer launch");
+    }
     return PMO_OK;
 }
 

@@ -132,8 +132,14 @@ struct WalkParams {
     // LDS-privatised box of the absorbed tally around the launch voxel
     int32_t win_x0, win_y0, win_z0;
     uint32_t win_nx, win_ny, win_nz, win_cells;
+    // absorbed weight outside the box, gathered per 2x2x2-voxel cube (one 64 B line of FP64)
+    // in a per-lane LDS line buffer and added to `lines` (cube-major, 8 doubles per cube)
+    // one line per request; null: one global atomic per voxel deposit
+    double* lines;
+    uint32_t nly, nlz;         // cubes along y and z
+    uint32_t lbuf_stride;      // block size: element e of lane t is at lbuf[e * stride + (t ^ e)]
     // dynamic LDS layout (bytes)
-    uint32_t lds_tab, lds_scal, lds_ev, lds_hist, lds_win;
+    uint32_t lds_tab, lds_scal, lds_ev, lds_hist, lds_lbuf, lds_stage, lds_win;
     uint32_t h_rd, h_tt, h_top, h_rda, h_tta, h_line, h_ring, h_rz, h_total;  // offsets in doubles
 };
 
@@ -321,6 +327,51 @@ __device__ __forceinline__ int32_t clamp_index(int32_t a, uint32_t n)
     return a < 0 ? 0 : (a >= (int32_t)n ? (int32_t)n - 1 : a);
 }
 
+// ------------------------------------------------------------------ line-buffered deposits
+// A lane's line buffer holds the 8 FP64 sums of the 2x2x2 cube its packet last deposited in
+// outside the LDS box; element e of lane t lives at lbuf[line_slot(e, t)] (element-major,
+// lanes XOR-swizzled so that the eight lanes reading one buffer hit eight banks).
+// (the stride is the block size, passed as a kernel argument: blockDim is read from the
+// dispatch packet with a vector load, which would wait on every load in flight)
+__device__ __forceinline__ uint32_t line_slot(uint32_t e, uint32_t t, uint32_t stride) { return e * stride + (t ^ e); }
+
+// Hands the line buffers of the wave's lanes that `want` over to global memory, eight
+// buffers per round: the eight lanes 8k..8k+7 read the eight elements of the k-th buffer and
+// add the non-zero ones to its cube's line with one atomic instruction, so each buffer costs
+// one 64 B request; the readers clear what they read. Wave-uniform: every lane of the wave
+// must call it together. The stage (16 words per wave) passes buffer owners and lines.
+__device__ __forceinline__ void flush_lines(const WalkParams& P, double* lbuf, int32_t* stage, bool want,
+                                            int32_t& cur_line, unsigned long long lt_mask, uint32_t& n_req)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    int32_t* st = stage + (threadIdx.x >> 6) * 16u;
+    unsigned long long fm = __ballot(want);
+    while (fm != 0ull) {
+        const uint32_t nf = (uint32_t)__popcll(fm) < 8u ? (uint32_t)__popcll(fm) : 8u;
+        const uint32_t rank = (uint32_t)__popcll(fm & lt_mask);
+        const bool mine = want && rank < 8u;
+        if (mine) { st[rank] = (int32_t)threadIdx.x; st[8u + rank] = cur_line; }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t slot = lane >> 3, j = lane & 7u;
+        if (slot < nf) {
+            const uint32_t owner = (uint32_t)st[slot];
+            const uint32_t line = (uint32_t)st[8u + slot];
+            double* q = lbuf + line_slot(j, owner, P.lbuf_stride);
+            const double v = *q;
+            *q = 0.0;
+            if (v != 0.0) gmem_add(P.lines + (size_t)line * 8u + j, v);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (mine) { cur_line = -1; want = false; }
+        if (lane == 0) n_req += nf;
+        fm = __ballot(want);
+    }
+}
+
 // ------------------------------------------------------------------ the walk kernel
 // RZ: also tally the r-z fluence (a separate instantiation, so the plain walk keeps its
 // register budget). SKIP: P.skip holds each voxel's uniform-cube radius, and a packet in a
@@ -329,8 +380,10 @@ __device__ __forceinline__ int32_t clamp_index(int32_t a, uint32_t n)
 // where the reference moves on without a draw (_kernel.py:82), so every packet makes the
 // same draws (and events, unless rounding flips a decision at a face); only the arithmetic
 // of the merged steps is rounded once instead of per face.
-template <typename R, bool RZ, bool SKIP>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) walk_kernel(const WalkParams P)
+// LINES: the absorbed map outside the LDS box goes through per-lane line buffers (P.lines);
+// a separate kernel with its own register budget (below), as it holds more per lane.
+template <typename R, bool RZ, bool SKIP, bool LINES>
+__device__ __forceinline__ void walk_body(const WalkParams& P)
 {
     using M = Mth<R>;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -339,6 +392,9 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
     unsigned long long* bev = reinterpret_cast<unsigned long long*>(smem + P.lds_ev);
     double* hist = reinterpret_cast<double*>(smem + P.lds_hist);
     double* win = reinterpret_cast<double*>(smem + P.lds_win);
+    double* lbuf = reinterpret_cast<double*>(smem + P.lds_lbuf);
+    int32_t* stage = reinterpret_cast<int32_t*>(smem + P.lds_stage);
+    const bool use_lines = LINES && P.lines != nullptr;
 
     const uint32_t end = P.n_labels;
     for (uint32_t i = threadIdx.x; i <= end; i += blockDim.x) tab[i] = label_const<R>(P.optics[i]);
@@ -346,6 +402,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
     for (uint32_t i = threadIdx.x; i < kBE_COUNT; i += blockDim.x) bev[i] = 0ull;
     for (uint32_t i = threadIdx.x; i < P.h_total; i += blockDim.x) hist[i] = 0.0;
     for (uint32_t i = threadIdx.x; i < P.win_cells; i += blockDim.x) win[i] = 0.0;
+    if (use_lines)
+        for (uint32_t i = threadIdx.x; i < 8u * blockDim.x; i += blockDim.x) lbuf[i] = 0.0;
     __syncthreads();
 
     const uint32_t lane = threadIdx.x & 63u;
@@ -372,12 +430,26 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
     double dep_acc = 0.0;
     uint32_t n_cross = 0, n_glob = 0;
     uint32_t n_pk = 0;               // packets this wave launched (wave-uniform)
+    // line buffer: the cube it holds (-1 none) and a deposit into another cube waiting for
+    // the buffer's hand-over at the top of the next iteration (p_line -1 none)
+    int32_t cur_line = -1, p_line = -1;
+    uint32_t p_e = 0;
+    double p_dep = 0.0;
 
     // wave-uniform work batch
     unsigned long long b_next = 0, b_end = 0;
     bool exhausted = false;
 
     for (;;) {
+        // ---------------------------------------------------- line buffers (all lanes here)
+        if (use_lines) {
+            flush_lines(P, lbuf, stage, p_line >= 0 && cur_line >= 0, cur_line, lt_mask, n_glob);
+            if (p_line >= 0) {
+                cur_line = p_line;
+                lds_add(lbuf + line_slot(p_e, threadIdx.x, P.lbuf_stride), p_dep);
+                p_line = -1;
+            }
+        }
         // ---------------------------------------------------------- refill idle lanes
         const unsigned long long need = __ballot(!alive);
         if (need != 0ull && !exhausted) {
@@ -589,17 +661,31 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
                 if (P.absorbed) {
                     const uint32_t lx = (uint32_t)(ax - P.win_x0), ly = (uint32_t)(ay - P.win_y0),
                                    lz = (uint32_t)(az - P.win_z0);
-                    int32_t slot;
-                    if (lx < P.win_nx && ly < P.win_ny && lz < P.win_nz)
-                        slot = -(int32_t)((lx * P.win_ny + ly) * P.win_nz + lz) - 1;
-                    else
-                        slot = (int32_t)((__umul24((uint32_t)ax, P.ny) + (uint32_t)ay) * P.nz + (uint32_t)az);
-                    if (slot == dep_slot) {
-                        dep_acc += dep;
+                    const bool in_win = lx < P.win_nx && ly < P.win_ny && lz < P.win_nz;
+                    if (use_lines && !in_win) {
+                        // into the lane's line buffer if the cube is the one it holds, else
+                        // wait for the hand-over of the buffer at the next iteration
+                        const int32_t li = (int32_t)((__umul24((uint32_t)ax >> 1, P.nly) + ((uint32_t)ay >> 1)) * P.nlz +
+                                                     ((uint32_t)az >> 1));
+                        const uint32_t e = (((uint32_t)ax & 1u) << 2) | (((uint32_t)ay & 1u) << 1) | ((uint32_t)az & 1u);
+                        if (li == cur_line) {
+                            lds_add(lbuf + line_slot(e, threadIdx.x, P.lbuf_stride), dep);
+                        } else {
+                            p_line = li; p_e = e; p_dep = dep;
+                        }
                     } else {
-                        flush_deposit(P, win, dep_slot, dep_acc, n_glob);
-                        dep_slot = slot;
-                        dep_acc = dep;
+                        int32_t slot;
+                        if (in_win)
+                            slot = -(int32_t)((lx * P.win_ny + ly) * P.win_nz + lz) - 1;
+                        else
+                            slot = (int32_t)((__umul24((uint32_t)ax, P.ny) + (uint32_t)ay) * P.nz + (uint32_t)az);
+                        if (slot == dep_slot) {
+                            dep_acc += dep;
+                        } else {
+                            flush_deposit(P, win, dep_slot, dep_acc, n_glob);
+                            dep_slot = slot;
+                            dep_acc = dep;
+                        }
                     }
                 }
                 if (w <= (R)P.wth) {                               // roulette (:132-141)
@@ -663,7 +749,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
                 const float x = P.pitch * (float)(ax - P.cx) + (float)px;
                 const float y = P.pitch * (float)(ay - P.cy) + (float)py;
                 const double z64 = (double)P.pitch * (double)(az - 1) + (double)pz + 0.5 * (double)P.pitch;
-                const float z = (float)(__builtin_rint(z64 * 1e6) / 1e6);
+                const float z = (float)(__builtin_rint(z64 * 1e6) * 1e-6);
                 const float fvz = (float)vz;
                 lds_add(&bscal[fvz < 0.f ? kBS_RD : (fvz > 0.f ? kBS_TT : kBS_OTHER)], wd);
                 lds_add(&bscal[kBS_FACE + 2 * xk + (xs > 0 ? 1 : 0)], wd);
@@ -720,6 +806,16 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
     }
 
     // ---------------------------------------------------------------- block flush
+    if (use_lines) {
+        // the deposits still waiting, then every line buffer (all lanes are here)
+        flush_lines(P, lbuf, stage, p_line >= 0 && cur_line >= 0, cur_line, lt_mask, n_glob);
+        if (p_line >= 0) {
+            cur_line = p_line;
+            lds_add(lbuf + line_slot(p_e, threadIdx.x, P.lbuf_stride), p_dep);
+            p_line = -1;
+        }
+        flush_lines(P, lbuf, stage, cur_line >= 0, cur_line, lt_mask, n_glob);
+    }
     if (P.absorbed) flush_deposit(P, win, dep_slot, dep_acc, n_glob);
     lds_add(&bscal[kBS_ABS], acc_abs);
     if (lane == 0) {
@@ -758,6 +854,21 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) 
     }
 }
 
+template <typename R, bool RZ, bool SKIP>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) walk_kernel(const WalkParams P)
+{
+    walk_body<R, RZ, SKIP, false>(P);
+}
+
+// the FP32 walk with line-buffered absorbed deposits: it runs at 4 waves per SIMD (one
+// 1024-lane block per CU, or four 256-lane blocks), whose register budget holds the line
+// buffer's bookkeeping without spills
+template <bool SKIP>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) walk_lines_kernel(const WalkParams P)
+{
+    walk_body<float, false, SKIP, true>(P);
+}
+
 // Fluence2D / Fluence3D binning of points (pymopt/fluence/intarnal_fluence.py)
 __global__ void __launch_bounds__(256) fluence_bin_kernel(const float* p, unsigned long long stride, const float* w,
                                                           unsigned long long n, int dims, int nr, float dr, int nz,
@@ -786,6 +897,34 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n)
 {
     const int i = threadIdx.x;
     if (i < n) p[i] = 0ull;
+}
+
+// clears n doubles (a fill kernel, so the walk can be graph-captured)
+__global__ void __launch_bounds__(256) zero_f64_kernel(double* p, unsigned long long n)
+{
+    const unsigned long long step = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) p[i] = 0.0;
+}
+
+// absorbed[x][y][z] += lines[cube of (x, y, z)][element]: the cube-major line sums into the
+// C-order map; one thread per cube, its 8 elements to the up-to-8 voxels inside the volume
+__global__ void __launch_bounds__(256) lines_to_absorbed_kernel(const double* lines, double* absorbed, uint32_t nx,
+                                                                uint32_t ny, uint32_t nz, uint32_t nly, uint32_t nlz,
+                                                                unsigned long long n_cubes)
+{
+    const unsigned long long step = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long c = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; c < n_cubes; c += step) {
+        const uint32_t cz = (uint32_t)(c % nlz);
+        const unsigned long long t = c / nlz;
+        const uint32_t cy = (uint32_t)(t % nly), cx = (uint32_t)(t / nly);
+        const double* q = lines + c * 8u;
+        for (uint32_t e = 0; e < 8u; ++e) {
+            const double v = q[e];
+            if (v == 0.0) continue;
+            const uint32_t x = 2u * cx + (e >> 2), y = 2u * cy + ((e >> 1) & 1u), z = 2u * cz + (e & 1u);
+            if (x < nx && y < ny && z < nz) absorbed[((size_t)x * ny + y) * nz + z] += v;
+        }
+    }
 }
 
 // probes of the FP64 atomic rate and the FP32 FMA issue rate (every index in bounds)

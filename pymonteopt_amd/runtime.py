@@ -41,6 +41,7 @@ class TallySpec:
     line_y: float = 5.0
     rings: tuple = ()            # ((r_in, r_out), ...) mm, top-face annuli
     absorbed: bool = True
+    lines: bool = True           # gather the absorbed map per 2x2x2 cube (64 B line) before its atomics
     exit_state: bool = False
     exit_voxel: bool = False     # the reference kernel's exit state: end voxel + in-voxel position
     events: bool = True
@@ -50,12 +51,16 @@ class TallySpec:
 class TallySet:
     """FP64 device tally buffers for one volume; zero() before reuse."""
 
-    def __init__(self, spec: TallySpec, nvox: int, n_packets: int, device):
+    def __init__(self, spec: TallySpec, shape, n_packets: int, device):
         self.spec = spec
         f64 = dict(dtype=torch.float64, device=device)
+        nvox = int(shape[0]) * int(shape[1]) * int(shape[2])
         self.scalars = torch.zeros(_abi.PMO_S_COUNT, **f64)
         self.events = torch.zeros(_abi.PMO_EV_COUNT, dtype=torch.int64, device=device) if spec.events else None
         self.absorbed = torch.zeros(nvox, **f64) if spec.absorbed else None
+        # scratch of the line-gathered map (the library clears it and adds it into absorbed)
+        self.absorbed_lines = (torch.empty(_abi.absorbed_lines_doubles(*shape), **f64)
+                               if spec.absorbed and spec.lines else None)
         self.hist = {}
         if spec.nr > 0:
             for k in spec.radial:
@@ -110,6 +115,7 @@ class TallySet:
         t.exit_state = _ptr(self.exit_state)
         t.exit_add = _ptr(self.exit_add)
         t.exit_p = _ptr(self.exit_p)
+        t.absorbed_lines = _ptr(self.absorbed_lines)
         if s.rz:
             t.rz_nr, t.rz_dr, t.rz_nz, t.rz_dz = int(s.rz[0]), float(s.rz[1]), int(s.rz[2]), float(s.rz[3])
             t.fluence_rz = _ptr(self.hist["fluence_rz"])
@@ -217,7 +223,7 @@ class Walker:
         self.optics.copy_(o)
 
     def tallies(self, spec: TallySpec, n_packets: int = 0) -> TallySet:
-        return TallySet(spec, self.nvox, n_packets, self.device)
+        return TallySet(spec, self.shape, n_packets, self.device)
 
     def volume_struct(self, skip=False) -> _abi.PmoVolume:
         """The volume for the C ABI; skip=True adds the uniform-cube radius (pmo_walk only)."""
@@ -301,6 +307,40 @@ class Walker:
         self._inflight = [(e, k) for e, k in self._inflight if not e.query()]
         if keep:
             self._inflight.append((ev, keep))
+
+    # launch shapes tried by tune(): the library default and the shapes that measured best
+    # on the MI355X for the skin plate (one 1024-lane block per CU with a 24x24x16 window)
+    # and the 256^3 bone phantom (five 256-lane blocks per CU with 9x9x21 windows)
+    TUNE_CANDIDATES = ({}, {"block_size": 1024, "window": (24, 24, 16), "quorum": 40},
+                       {"block_size": 256, "window": (9, 9, 21), "quorum": 32},
+                       {"block_size": 256, "window": (12, 12, 12), "quorum": 40})
+
+    def tune(self, src: Source, spec: TallySpec, n_packets=10_000_000, candidates=None, seed=0x7FFF0000):
+        """Times one walk of `n_packets` packets with each launch shape of `candidates`
+        (block size, LDS window, quorum) on this volume and tallies, HIP events on torch's
+        current stream, and returns the fastest as walk() keywords with its time. Every shape
+        walks the same packets with the same draws; only the speed differs."""
+        cands = self.TUNE_CANDIDATES if candidates is None else candidates
+        tal = self.tallies(spec, n_packets if (spec.exit_state or spec.exit_voxel) else 0)
+        best = None
+        for kw in cands:
+            try:
+                self.walk(src, max(1, n_packets // 10), seed, tal, **kw)     # code objects, LDS shape
+            except (ValueError, _abi.PmoError):
+                continue                                                     # shape does not fit
+            tal.zero()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            st = torch.cuda.current_stream(self.device)
+            e0.record(st)
+            self.walk(src, n_packets, seed, tal, **kw)
+            e1.record(st)
+            e1.synchronize()
+            t = e0.elapsed_time(e1) * 1e-3
+            if best is None or t < best[1]:
+                best = (dict(kw), t)
+        if best is None:
+            raise ValueError("no launch shape fits this walk")
+        return best
 
     def walk(self, src: Source, n_packets: int, seed: int, tallies: TallySet, *, hip_stream=None, **run):
         """Launch one walk (asynchronous on torch's current stream unless a HIP stream handle

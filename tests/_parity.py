@@ -49,3 +49,9 @@ def binned_z(A, B):
 def ledger_error(s):
     bal = s["Rd"] + s["Tt"] + s["other"] + s["A"] + s["kill"] - s["gain"]
     return abs(s["w0"] - bal) / s["w0"]
+
+
+def packet_events(r):
+    """The per-packet event counts of a run: every counter except the atomic-request count,
+    which depends on how packets shared lanes (and their deposit buffers) in turn."""
+    return {k: v for k, v in r["events"].items() if k != "global_deposits"}

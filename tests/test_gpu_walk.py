@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from _parity import binned_z, gpu_run, ledger_error, oracle_run, zscore
+from _parity import binned_z, gpu_run, ledger_error, oracle_run, packet_events, zscore
 
 pytestmark = pytest.mark.gpu
 
@@ -323,13 +323,15 @@ def test_whole_volume_window_is_invisible(torch_cuda):
 
 def test_event_scheduling_is_invisible(slab):
     """Which step a wave runs first (quorum) changes only the interleaving of lanes, not
-    any packet's own event sequence: tallies agree to FP64 summation order."""
+    any packet's own event sequence: tallies agree to FP64 summation order. (The count of
+    atomic requests is not a per-packet event: it follows which packets share a lane's
+    deposit buffers in turn.)"""
     wl, wk = slab
     N = 50000
     base = gpu_run(wk, wl, N, 13, quorum=1)
     for q in (16, 48, 64):
         r = gpu_run(wk, wl, N, 13, quorum=q)
-        assert r["events"] == base["events"]
+        assert packet_events(r) == packet_events(base)
         assert np.allclose(r["absorbed"], base["absorbed"], rtol=1e-10, atol=1e-15)
         for k in ("Rd", "Tt", "A"):
             assert r["scalars"][k] == pytest.approx(base["scalars"][k], rel=1e-12)
@@ -445,7 +447,7 @@ def test_per_packet_launch_states(slab, oracle):
     a = gpu_run(wk, wl, N, 21)
     st = dataclasses.replace(wl, source=dataclasses.replace(wl.source, states=_pencil_states(wl, N)))
     b = gpu_run(wk, st, N, 21)
-    assert a["events"] == b["events"]
+    assert packet_events(a) == packet_events(b)
     for k in ("Rd", "Tt", "A", "w0"):
         assert b["scalars"][k] == pytest.approx(a["scalars"][k], rel=1e-12), k
     assert np.allclose(a["absorbed"], b["absorbed"], rtol=1e-10, atol=1e-15)
@@ -479,6 +481,30 @@ def test_compat_takes_per_packet_launch_arrays(slab):
     r = gpu_run(wk, tem, N, 8)
     assert w_o[v_o[2] < 0].sum(dtype=np.float64) == pytest.approx(r["scalars"]["Rd"], rel=1e-6)
     assert w_o[v_o[2] > 0].sum(dtype=np.float64) == pytest.approx(r["scalars"]["Tt"], rel=1e-6)
+
+
+@pytest.mark.parametrize("n_packets", [200000, 1200000])
+def test_line_buffered_deposits_are_invisible(torch_cuda, n_packets):
+    """The absorbed map gathered per 2x2x2 cube in per-lane line buffers (absorbed_lines)
+    equals the map deposited voxel by voxel, on the same seeds, with the regular launch
+    (2e5 packets) and the wide one (1.2e6: 1024-lane blocks, LDS window shrunk beside the
+    line buffers); the walk is otherwise the same, and it issues fewer atomic requests."""
+    from pymonteopt_amd import presets
+    from pymonteopt_amd.runtime import TallySpec, Walker
+    wl = presets.skin_plate()
+    wk = Walker(wl.labels, wl.pitch, wl.optics)
+    a = gpu_run(wk, wl, n_packets, 17, spec=TallySpec(**wl.tallies, lines=True))
+    b = gpu_run(wk, wl, n_packets, 17, spec=TallySpec(**wl.tallies, lines=False))
+    for k in ("packets", "interactions", "crossings", "fresnel", "roulette", "killed", "escaped"):
+        assert a["events"][k] == b["events"][k], k
+    for k, v in b["scalars"].items():
+        assert a["scalars"][k] == pytest.approx(v, rel=1e-12, abs=1e-12), k
+    assert np.allclose(a["absorbed"], b["absorbed"], rtol=1e-9, atol=1e-13)
+    assert a["absorbed"].sum() == pytest.approx(a["scalars"]["A"], rel=1e-12)
+    assert np.allclose(a["rd_r"], b["rd_r"], rtol=1e-12, atol=0)
+    print(f"{n_packets} packets: atomic requests per packet {a['events']['global_deposits'] / n_packets:.1f} "
+          f"(lines) vs {b['events']['global_deposits'] / n_packets:.1f} (voxel by voxel)", flush=True)
+    assert a["events"]["global_deposits"] < b["events"]["global_deposits"]
 
 
 def test_walks_of_one_walker_on_two_streams(slab):
@@ -520,7 +546,7 @@ def test_packet_range_shards_sum_to_the_whole_run(slab):
         wk.walk(wl.source, b - a, 7, tal, first_packet=a)
     torch.cuda.synchronize()
     parts = tal.to_host(shape=wk.shape)
-    assert parts["events"] == whole["events"]
+    assert packet_events(parts) == packet_events(whole)
     for k in ("Rd", "Tt", "A", "w0"):
         assert parts["scalars"][k] == pytest.approx(whole["scalars"][k], rel=1e-12), k
     assert np.allclose(parts["absorbed"], whole["absorbed"], rtol=1e-10, atol=1e-15)
