@@ -137,10 +137,11 @@ def batched_leg(n_inst=512):
 
     # layer stepping (the default) and voxel stepping, each autotuned, graph-captured
     sw = PlateSweep(plates, nr=40, dr=0.1, batch=256)
-    tuned = autotune.tune_sweep(sw, blocks=(128, 256), batches=(128, 256, 512), quorums=(32, 24, 48))
+    tuned = autotune.tune_sweep(sw, blocks=(128, 256), batches=(128, 256, 512), quorums=(32, 24, 48), refills=(4, 2, 8))
     tb, rb = graph_time(sw)
     swv = PlateSweep(plates, nr=40, dr=0.1, batch=256, layer_step=False)
-    tuned_v = autotune.tune_sweep(swv, blocks=(128, 256), batches=(128, 256, 512), quorums=(32, 24, 48))
+    tuned_v = autotune.tune_sweep(swv, blocks=(128, 256), batches=(128, 256, 512), quorums=(32, 24, 48),
+                                  refills=(4, 2, 8))
     tv, rv = graph_time(swv)
     spec = TallySpec(nr=40, dr=0.1, absorbed=False)
 

@@ -155,7 +155,11 @@ typedef struct pmo_run {
                                 non-NULL, so a captured HIP graph can be replayed with new keys */
     int32_t quorum;          /* a wave runs its interaction step once this many of its 64 lanes are
                                 ready for one (else it advances face crossings); 0 = default
-                                (32)                                                            */
+                                (32; 40 with absorbed_lines)                                    */
+    int32_t refill;          /* pmo_walk, pmo_walk_plates: a wave starts new packets once this many
+                                of its lanes are idle (or all are), so the launch code runs for
+                                several lanes at a time; 1 = every idle lane at once; 0 = default
+                                (4). Packets and their histories do not change.                 */
 } pmo_run;
 
 typedef struct pmo_tallies {

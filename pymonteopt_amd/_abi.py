@@ -58,7 +58,8 @@ class PmoRun(C.Structure):
                 ("seed", C.c_uint32), ("stream", C.c_uint32),
                 ("wth", C.c_float), ("roulette_m", C.c_float), ("flags", C.c_uint32),
                 ("block_size", C.c_int32), ("grid_blocks", C.c_int32), ("packets_per_lane", C.c_int32),
-                ("window", C.c_int32 * 3), ("key_dev", C.c_void_p), ("quorum", C.c_int32)]
+                ("window", C.c_int32 * 3), ("key_dev", C.c_void_p), ("quorum", C.c_int32),
+                ("refill", C.c_int32)]
 
 
 class PmoTallies(C.Structure):

@@ -81,11 +81,11 @@ def tune_walk(walker, source, spec, n_packets, *, sample=2_000_000, blocks=(128,
     return res
 
 
-def tune_sweep(sweep, *, blocks=(128, 256, 512, 1024), batches=(64, 128, 256, 512, 1024), quorums=(32,), seed=0,
-               cache_path=None):
-    """Best (block_size, batch, quorum) for a PlateSweep, searched one after the other
-    (block and batch together, then the quorum at the best pair); leaves the sweep set to
-    the winner."""
+def tune_sweep(sweep, *, blocks=(128, 256, 512, 1024), batches=(64, 128, 256, 512, 1024), quorums=(32,),
+               refills=(4,), seed=0, cache_path=None):
+    """Best (block_size, batch, quorum, refill) for a PlateSweep, searched one after the
+    other (block and batch together, then the quorum at the best pair, then the refill
+    threshold); leaves the sweep set to the winner."""
     dev = torch.cuda.get_device_name(sweep.device)
     sig = tuple((tuple(p.thickness), tuple(p.ma), tuple(p.ms), p.n_packets) for p in sweep.plates[:8])
     key = f"sweep|{dev}|{sweep.n_inst}|{sweep.total_packets}|{sweep.layer_step}|{hash(sig)}"
@@ -95,30 +95,38 @@ def tune_sweep(sweep, *, blocks=(128, 256, 512, 1024), batches=(64, 128, 256, 51
         res = _load(cache_path).get(key)
     if res is None:
         table = []
-        orig_batch, orig_q = sweep.batch, sweep.quorum
-        q0 = quorums[0]
-        sweep.quorum = q0
+        orig_batch, orig_q, orig_r = sweep.batch, sweep.quorum, sweep.refill
+        q0, r0 = quorums[0], refills[0]
+        sweep.quorum, sweep.refill = q0, r0
         for b in batches:
             sweep.set_batch(b)
             for bs in blocks:
                 t = _time(lambda: sweep.run(seed, block_size=bs), reps=1)
-                table.append({"block_size": bs, "batch": b, "quorum": q0, "seconds": t,
+                table.append({"block_size": bs, "batch": b, "quorum": q0, "refill": r0, "seconds": t,
                               "photons_per_s": sweep.total_packets / t})
         best = min(table, key=lambda r: r["seconds"])
         sweep.set_batch(best["batch"])
         for q in quorums[1:]:
             sweep.quorum = q
             t = _time(lambda: sweep.run(seed, block_size=best["block_size"]), reps=1)
-            table.append({"block_size": best["block_size"], "batch": best["batch"], "quorum": q, "seconds": t,
-                          "photons_per_s": sweep.total_packets / t})
+            table.append({"block_size": best["block_size"], "batch": best["batch"], "quorum": q, "refill": r0,
+                          "seconds": t, "photons_per_s": sweep.total_packets / t})
+        best = min(table, key=lambda r: r["seconds"])
+        sweep.quorum = best["quorum"]
+        for rf in refills[1:]:
+            sweep.refill = rf
+            t = _time(lambda: sweep.run(seed, block_size=best["block_size"]), reps=1)
+            table.append({"block_size": best["block_size"], "batch": best["batch"], "quorum": best["quorum"],
+                          "refill": rf, "seconds": t, "photons_per_s": sweep.total_packets / t})
         best = min(table, key=lambda r: r["seconds"])
         sweep.set_batch(orig_batch)
-        sweep.quorum = orig_q
+        sweep.quorum, sweep.refill = orig_q, orig_r
         res = {"block_size": best["block_size"], "batch": best["batch"], "quorum": best["quorum"],
-               "photons_per_s": best["photons_per_s"], "table": table}
+               "refill": best["refill"], "photons_per_s": best["photons_per_s"], "table": table}
         _CACHE[key] = res
         _store(cache_path, key, res)
     sweep.set_batch(res["batch"])
     sweep.block_size = res["block_size"]
     sweep.quorum = res.get("quorum", 0)
+    sweep.refill = res.get("refill", 0)
     return res

@@ -158,6 +158,7 @@ class PlateSweep:
         self.key = torch.zeros(2, dtype=torch.int32, device=self.device)
         self.block_size = 0          # 0 = library default; autotune.tune_sweep sets the best
         self.quorum = 0              # lanes ready before a wave runs interactions; 0 = library default
+        self.refill = 0              # idle lanes a wave gathers before starting packets; 0 = library default
         self._graph = None
 
     def set_batch(self, batch):
@@ -193,6 +194,7 @@ class PlateSweep:
         r.flags = (_abi.PMO_F_NORMAL_FRESNEL if normal_fresnel else 0) | (_abi.PMO_F_LAYER_STEP if self.layer_step else 0)
         r.block_size = int(block_size)
         r.quorum = int(self.quorum)
+        r.refill = int(self.refill)
         r.key_dev = C.c_void_p(self.key.data_ptr()) if key_dev else None
         t = _abi.PmoBatchTallies()
         t.scalars = C.c_void_p(self.scalars.data_ptr())

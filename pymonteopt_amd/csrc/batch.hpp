@@ -39,6 +39,7 @@ struct BatchParams {
     uint32_t flags;
     uint32_t max_events;
     uint32_t quorum;            // lanes wanting an interaction before a wave runs that step, >= 1
+    uint32_t refill;            // idle lanes before a wave starts new packets (all idle: always)
     double* scalars;            // [n_inst][kBS_COUNT]
     unsigned long long* events; // [kBE_COUNT]
     int32_t nr;
@@ -114,7 +115,7 @@ plate_batch_kernel(const BatchParams P)
 
     for (;;) {
         const unsigned long long need = __ballot(!alive);
-        if (need != 0ull && !exhausted) {
+        if (need != 0ull && !exhausted && ((uint32_t)__popcll(need) >= P.refill || need == ~0ull)) {
             if (b_next == b_end) {
                 unsigned long long claim = 0;
                 if (lane == 0) claim = atomicAdd(P.work, 1ull);

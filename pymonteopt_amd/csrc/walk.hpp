@@ -102,6 +102,7 @@ struct WalkParams {
     uint32_t batch;            // packet ids claimed per wave per counter atomic
     uint32_t max_events;       // per-packet event guard
     uint32_t quorum;           // lanes ready for an interaction before a wave runs that step
+    uint32_t refill;           // idle lanes before a wave starts new packets (all idle: always)
     // tallies
     double* scalars;
     unsigned long long* events;
@@ -452,7 +453,7 @@ __device__ __forceinline__ void walk_body(const WalkParams& P)
         }
         // ---------------------------------------------------------- refill idle lanes
         const unsigned long long need = __ballot(!alive);
-        if (need != 0ull && !exhausted) {
+        if (need != 0ull && !exhausted && ((uint32_t)__popcll(need) >= P.refill || need == ~0ull)) {
             if (b_next == b_end) {
                 unsigned long long claim = 0;
                 if (lane == 0) claim = atomicAdd(P.work, (unsigned long long)P.batch);

@@ -252,7 +252,7 @@ class Walker:
 
     def c_args(self, src: Source, n_packets: int, seed: int, tallies: TallySet, *, stream_id=0,
                first_packet=0, wth=1e-4, roulette_m=10.0, fp64=False, normal_fresnel=False,
-               block_size=0, grid_blocks=0, packets_per_lane=0, window=(0, 0, 0), quorum=0):
+               block_size=0, grid_blocks=0, packets_per_lane=0, window=(0, 0, 0), quorum=0, refill=0):
         """The pmo_volume, pmo_source, pmo_run and pmo_tallies of one walk, plus the device
         launch states that must stay alive until the walk has read them."""
         if tallies.n_exit and tallies.n_exit != int(n_packets):
@@ -285,6 +285,7 @@ class Walker:
         for i in range(3):
             r.window[i] = int(window[i])
         r.quorum = int(quorum)
+        r.refill = int(refill)
         return vol, s, r, tallies.c_struct(), keep
 
     def launch_stream(self, hip_stream=None):
@@ -346,7 +347,7 @@ class Walker:
         """Launch one walk (asynchronous on torch's current stream unless a HIP stream handle
         is given; after this Walker's previous launch, whatever its stream). `run` takes the
         keywords of c_args: stream_id, first_packet, wth, roulette_m, fp64, normal_fresnel,
-        block_size, grid_blocks, packets_per_lane, window, quorum."""
+        block_size, grid_blocks, packets_per_lane, window, quorum, refill."""
         vol, s, r, t, keep = self.c_args(src, n_packets, seed, tallies, **run)
         with torch.cuda.device(self.device):
             st = self.launch_stream(hip_stream)
