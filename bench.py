@@ -183,7 +183,10 @@ def adjoint_leg(n_packets=10_000_000):
       forward with Jacobians (the backward is J^T g). The per-voxel replay of the same
       plate is reported beside it;
     * 64^3 4-label grid, per-voxel d Rd(r) / d mu_a as rd_of_absorption_field computes it:
-      a forward with the absorption field and the replay adjoint."""
+      a forward with the absorption field and the replay adjoint.
+    The replay's deposits outside its LDS box go either through cube line buffers or one FP64
+    atomic per voxel deposit beside a larger box; DiffWalk's first adjoint times both and keeps
+    the faster (the timed calls run after that choice). Both times are reported."""
     import numpy as np
     import torch
     from pymonteopt_amd import presets
@@ -200,6 +203,15 @@ def adjoint_leg(n_packets=10_000_000):
         e1.synchronize()
         return r, e0.elapsed_time(e1) * 1e-3
 
+    def replay_modes(fwd):
+        # the replay with each deposit path, then back to the chosen one
+        chosen, t = dw.lines, {}
+        for mode, key in ((True, "line_buffers"), (False, "per_voxel_atomics")):
+            dw.lines = mode
+            t[key] = timed(lambda: dw.adjoint(fwd, dw.area.clone()))[1]
+        dw.lines = chosen
+        return dict(t, chosen="line_buffers" if chosen else "per_voxel_atomics")
+
     for name, wl in (("skin", presets.skin_plate()), ("grf64", presets.grf_grid(seed=0))):
         dw = DiffWalk(wl.labels, wl.pitch, wl.optics, wl.source, n_packets=n_packets, nr=40, dr=0.1)
         wk = Walker(wl.labels, wl.pitch, wl.optics)
@@ -212,7 +224,8 @@ def adjoint_leg(n_packets=10_000_000):
             _, t_adj = timed(lambda: dw.adjoint(fwd, dw.area.clone()))
             t_grad = t_fwd
             rec = {"gradient": "per-label mu_a, mu_s: forward with Jacobians", "forward_with_jacobian_s": t_fwd,
-                   "per_voxel_replay_s": t_adj, "per_voxel_gradient_cost_vs_plain": (t_fwd + t_adj) / t_plain}
+                   "per_voxel_replay_s": t_adj, "per_voxel_replay_modes_s": replay_modes(fwd),
+                   "per_voxel_gradient_cost_vs_plain": (t_fwd + t_adj) / t_plain}
         else:
             mua = torch.tensor(wl.optics[:, 0], device="cuda:0")
             field = mua[torch.as_tensor(np.minimum(wl.labels, wl.n_labels), device="cuda:0").long()].float()
@@ -220,7 +233,7 @@ def adjoint_leg(n_packets=10_000_000):
             _, t_adj = timed(lambda: dw.adjoint(fwd, dw.area.clone()))
             t_grad = t_fwd + t_adj
             rec = {"gradient": "per-voxel mu_a: forward + replay adjoint", "forward_s": t_fwd,
-                   "adjoint_replay_s": t_adj}
+                   "adjoint_replay_s": t_adj, "adjoint_replay_modes_s": replay_modes(fwd)}
         rec.update({"plain_walk_s": t_plain, "gradient_s": t_grad, "gradient_cost_vs_plain": t_grad / t_plain,
                     "plain_walk_voxel_stepping_s": t_plain_v,
                     "gradient_cost_vs_voxel_stepping_plain": t_grad / t_plain_v,

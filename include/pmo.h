@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define PMO_ABI_VERSION 7
+#define PMO_ABI_VERSION 8
 
 /* ---- error codes ---------------------------------------------------------------- */
 #define PMO_OK 0
@@ -258,6 +258,13 @@ typedef struct pmo_diff {
     double* grad_mua;        /* adjoint: device [nx*ny*nz] accumulates d loss / d mu_a per voxel    */
     uint64_t* mismatch;      /* adjoint: device counter of replayed exits that differ from the
                                 forward record (0 when replay is exact), or NULL                    */
+    double* grad_lines;      /* adjoint: device scratch of PMO_ABSORBED_LINES_DOUBLES(nx, ny, nz)
+                                doubles, or NULL. With it the replay gathers its deposits outside
+                                the LDS box per 2x2x2-voxel cube in per-lane LDS buffers (one
+                                atomic request per 64 B cube line handed over) and then adds the
+                                scratch into grad_mua; the library clears it. NULL: one FP64
+                                atomic per voxel deposit. Same sums either way (up to FP64
+                                summation order).                                                   */
 } pmo_diff;
 
 /* ---- library ------------------------------------------------------------------------ */

@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpmo.so")
 
-ABI_VERSION = 7          # PMO_ABI_VERSION of include/pmo.h
+ABI_VERSION = 8          # PMO_ABI_VERSION of include/pmo.h
 
 PMO_OK = 0
 PMO_ERR_INVALID = -1
@@ -112,7 +112,7 @@ class PmoTuringParams(C.Structure):
 class PmoDiff(C.Structure):
     _fields_ = [("mua_field", C.c_void_p), ("nr", C.c_int32), ("dr", C.c_float), ("rd_r", C.c_void_p),
                 ("jac", C.c_void_p), ("exit_rec", C.c_void_p), ("coef", C.c_void_p), ("grad_mua", C.c_void_p),
-                ("mismatch", C.c_void_p)]
+                ("mismatch", C.c_void_p), ("grad_lines", C.c_void_p)]
 
 
 # every symbol include/pmo.h declares, with its ctypes signature

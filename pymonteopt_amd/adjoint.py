@@ -35,7 +35,8 @@ class DiffWalk:
     """A label volume ready for differentiable walks of `n_packets` packets with Rd(r)
     detected in `nr` bins of `dr` mm."""
 
-    def __init__(self, labels, pitch, optics, source: Source, n_packets=1e7, nr=40, dr=0.1, device="cuda:0"):
+    def __init__(self, labels, pitch, optics, source: Source, n_packets=1e7, nr=40, dr=0.1, device="cuda:0",
+                 lines="auto"):
         self.walker = Walker(labels, pitch, optics, device=device, skip=False)   # replay steps voxel by voxel
         self.device = self.walker.device
         self.base_optics = np.array(optics, np.float32)
@@ -47,6 +48,17 @@ class DiffWalk:
                                  device=self.device)
         self.lib = _abi.lib()
         self.last_mismatch = 0
+        # True: replay deposits outside the LDS box gather per 2x2x2-voxel cube in LDS line
+        # buffers and reach the gradient through a cube-major scratch (one atomic request per
+        # cube line); False: one FP64 atomic per voxel deposit, beside a larger LDS box.
+        # "auto": the first adjoint times both and keeps the faster (the line buffers pay
+        # where many deposits fall outside the box; on the 64^3 grid at 0.1 mm the larger box
+        # wins). Launches on a DiffWalk are ordered (Walker.launch_stream), so successive
+        # replays share the scratch.
+        if lines not in (True, False, "auto"):
+            raise ValueError("lines must be True, False or 'auto'")
+        self.lines = lines
+        self._grad_lines = None
 
     @property
     def shape(self):
@@ -87,7 +99,7 @@ class DiffWalk:
             field = mua_field.detach().to(self.device, torch.float32).contiguous()
             if tuple(field.shape) != tuple(self.shape):
                 raise ValueError(f"mua_field must have the padded volume shape {self.shape}")
-        d = PmoDiff(_p(field), self.nr, self.dr, _p(rd), _p(J), _p(rec), None, None, None)
+        d = PmoDiff(_p(field), self.nr, self.dr, _p(rd), _p(J), _p(rec), None, None, None, None)
         v, s, r = self._structs(seed, stream_id)
         with torch.cuda.device(self.device):
             st = self.walker.launch_stream()
@@ -112,12 +124,33 @@ class DiffWalk:
         if the walker's table changed since (another forward on this DiffWalk), and raises
         if any replayed packet left differently from its recorded exit: the gradient would
         then not be that of the forward pass."""
+        if self.lines == "auto":
+            self._line_scratch()
+            times = {}
+            for mode in (True, False):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                grad = self._replay(fwd, grad_rd_r, mode)
+                e1.record()
+                e1.synchronize()
+                times[mode] = e0.elapsed_time(e1)
+            self.lines = min(times, key=times.get)
+            return grad
+        return self._replay(fwd, grad_rd_r, self.lines)
+
+    def _line_scratch(self):
+        if self._grad_lines is None:
+            self._grad_lines = torch.empty(_abi.absorbed_lines_doubles(*self.shape), dtype=torch.float64,
+                                           device=self.device)
+        return self._grad_lines
+
+    def _replay(self, fwd, grad_rd_r, lines):
         g = grad_rd_r.detach().to(self.device, torch.float64).reshape(-1)
         coef = (g / (self.area * self.n_packets)).contiguous()
         grad = torch.zeros(self.shape, dtype=torch.float64, device=self.device)
         mism = torch.zeros(1, dtype=torch.int64, device=self.device)
         d = PmoDiff(_p(fwd["field"]), self.nr, self.dr, None, None, _p(fwd["exit_rec"]), _p(coef), _p(grad),
-                    _p(mism))
+                    _p(mism), _p(self._line_scratch()) if lines else None)
         v, s, r = self._structs(fwd["seed"], fwd["stream"])
         current = self.walker.optics.clone()
         self.walker.optics.copy_(fwd["optics"])              # stream-ordered before the replay

@@ -15,6 +15,9 @@
 //    (no stored paths); the replayed exit is checked against the record.
 // Optionally mu_a is a per-voxel field instead of the label's value, which makes the
 // per-voxel gradient the derivative of this forward with respect to that field.
+// The replay's deposits outside its LDS box may go through the walk's per-lane line buffers
+// (one 2x2x2-voxel cube each, handed over eight lanes per atomic instruction) into a
+// cube-major scratch that is added to the gradient map afterwards.
 #pragma once
 
 #include "walk.hpp"
@@ -49,12 +52,17 @@ struct DiffParams {
     double* grad;                  // [nx*ny*nz] d loss / d mu_a per voxel (replay)
     unsigned long long* mismatch;  // replayed exits that differ from the record
     unsigned long long* work;
+    double* lines;                 // replay: cube-major line scratch (8 doubles per 2x2x2 cube) or null
+    uint32_t nly, nlz;             // cubes along y and z
+    uint32_t lbuf_stride;          // block size (see line_slot)
     int32_t win_x0, win_y0, win_z0;
     uint32_t win_nx, win_ny, win_nz, win_cells;
-    uint32_t lds_tab, lds_scal, lds_ev, lds_hist, lds_jac, lds_win;
+    uint32_t lds_tab, lds_scal, lds_ev, lds_hist, lds_jac, lds_lbuf, lds_stage, lds_win;
 };
 
-template <typename R>
+// LINES: the replay with line buffers, an instantiation of its own so that the forward keeps
+// its register budget
+template <typename R, bool LINES>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) diff_kernel(const DiffParams P)
 {
     using M = Mth<R>;
@@ -65,16 +73,21 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
     double* hist = reinterpret_cast<double*>(smem + P.lds_hist);
     double* hjac = reinterpret_cast<double*>(smem + P.lds_jac);
     double* win = reinterpret_cast<double*>(smem + P.lds_win);
+    double* lbuf = reinterpret_cast<double*>(smem + P.lds_lbuf);
+    int32_t* stage = reinterpret_cast<int32_t*>(smem + P.lds_stage);
 
     const uint32_t end = P.n_labels;
     const uint32_t njac = P.jac ? P.n_labels : 0u;
     const bool replay = P.mode == 1;
+    const bool use_lines = LINES && replay && P.lines != nullptr;
     for (uint32_t i = threadIdx.x; i <= end; i += blockDim.x) tab[i] = label_const<R>(P.optics[i]);
     for (uint32_t i = threadIdx.x; i < kBS_COUNT; i += blockDim.x) bscal[i] = 0.0;
     for (uint32_t i = threadIdx.x; i < kBE_COUNT; i += blockDim.x) bev[i] = 0ull;
     for (uint32_t i = threadIdx.x; i < (uint32_t)P.nr; i += blockDim.x) hist[i] = 0.0;
     for (uint32_t i = threadIdx.x; i < (uint32_t)P.nr * njac * 2u; i += blockDim.x) hjac[i] = 0.0;
     for (uint32_t i = threadIdx.x; i < P.win_cells; i += blockDim.x) win[i] = 0.0;
+    if (use_lines)
+        for (uint32_t i = threadIdx.x; i < 8u * P.lbuf_stride; i += blockDim.x) lbuf[i] = 0.0;
     __syncthreads();
 
     const uint32_t lane = threadIdx.x & 63u;
@@ -93,13 +106,19 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
     uint32_t lab = 0, inter = 0, fres = 0, nev = 0;
     unsigned long long pid = 0;
     double acc_abs = 0.0;
-    // replay: pending gradient deposit (slot as in flush_deposit) and this packet's -c_b W
+    // replay: pending gradient deposit and this packet's -c_b W. The slot is -(box cell)-1 in
+    // the LDS box; outside it the voxel index, or with line buffers 8 x cube + element
     int32_t g_slot = kNoSlot;
     double g_len = 0.0, g_coef = 0.0;
     int32_t rec_bin = -1;
     float rec_w = 0.f;
-    uint32_t n_cross = 0;
+    uint32_t n_cross = 0, n_req = 0;
     uint32_t n_pk = 0;               // packets this wave launched (wave-uniform)
+    // line buffer: the cube it holds (-1 none) and a deposit into another cube waiting for
+    // the buffer's hand-over at the top of the next iteration (p_line -1 none)
+    int32_t cur_line = -1, p_line = -1;
+    uint32_t p_e = 0;
+    double p_dep = 0.0;
 
     unsigned long long b_next = 0, b_end = 0;
     bool exhausted = false;
@@ -122,35 +141,56 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
         L2 += l == 2 ? len : (R)0; N2 += l == 2 ? cnt : (R)0;
         L3 += l == 3 ? len : (R)0; N3 += l == 3 ? cnt : (R)0;
     };
+    auto grad_emit = [&](int32_t slot, double v, bool last) {
+        if (slot < 0) {
+            lds_add(win + (-slot - 1), v);
+        } else if (!use_lines) {
+            gmem_add(P.grad + slot, v);
+        } else {
+            const int32_t li = slot >> 3;
+            const uint32_t e = (uint32_t)slot & 7u;
+            if (li == cur_line) lds_add(lbuf + line_slot(e, threadIdx.x, P.lbuf_stride), v);
+            else if (!last) { p_line = li; p_e = e; p_dep = v; }
+            else gmem_add(P.lines + slot, v);
+        }
+    };
     auto grad_segment = [&](R len) {
         // replay: length `len` travelled in the current voxel
         const uint32_t lx = (uint32_t)(ax - P.win_x0), ly = (uint32_t)(ay - P.win_y0), lz = (uint32_t)(az - P.win_z0);
         int32_t slot;
         if (lx < P.win_nx && ly < P.win_ny && lz < P.win_nz)
             slot = -(int32_t)((lx * P.win_ny + ly) * P.win_nz + lz) - 1;
+        else if (use_lines)
+            slot = (int32_t)((((__umul24((uint32_t)ax >> 1, P.nly) + ((uint32_t)ay >> 1)) * P.nlz + ((uint32_t)az >> 1)) << 3) |
+                             (((uint32_t)ax & 1u) << 2) | (((uint32_t)ay & 1u) << 1) | ((uint32_t)az & 1u));
         else
             slot = (int32_t)((__umul24((uint32_t)ax, P.ny) + (uint32_t)ay) * P.nz + (uint32_t)az);
         if (slot == g_slot) {
             g_len += (double)len;
         } else {
-            if (g_slot != kNoSlot) {
-                if (g_slot < 0) lds_add(win + (-g_slot - 1), g_coef * g_len);
-                else gmem_add(P.grad + g_slot, g_coef * g_len);
-            }
+            if (g_slot != kNoSlot) grad_emit(g_slot, g_coef * g_len, false);
             g_slot = slot;
             g_len = (double)len;
         }
     };
     auto grad_flush = [&]() {
-        if (g_slot != kNoSlot) {
-            if (g_slot < 0) lds_add(win + (-g_slot - 1), g_coef * g_len);
-            else gmem_add(P.grad + g_slot, g_coef * g_len);
-        }
+        // the packet's last deposit: this iteration may already hold one for the hand-over,
+        // so a deposit outside the lane's current cube goes to its line directly
+        if (g_slot != kNoSlot) grad_emit(g_slot, g_coef * g_len, true);
         g_slot = kNoSlot;
         g_len = 0.0;
     };
 
     for (;;) {
+        if (use_lines) {
+            // line buffers (all lanes are here): hand over those whose lane waits to start another
+            flush_lines(P.lines, P.lbuf_stride, lbuf, stage, p_line >= 0 && cur_line >= 0, cur_line, lt_mask, n_req);
+            if (p_line >= 0) {
+                cur_line = p_line;
+                lds_add(lbuf + line_slot(p_e, threadIdx.x, P.lbuf_stride), p_dep);
+                p_line = -1;
+            }
+        }
         const unsigned long long need = __ballot(!alive);
         if (need != 0ull && !exhausted) {
             if (b_next == b_end) {
@@ -370,6 +410,10 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
         }
     }
 
+    if (use_lines) {
+        // every line buffer still held (no deposit waits: the loop's top placed the last)
+        flush_lines(P.lines, P.lbuf_stride, lbuf, stage, cur_line >= 0, cur_line, lt_mask, n_req);
+    }
     if (!replay) {
         lds_add(&bscal[kBS_ABS], acc_abs);
         if (lane == 0) {

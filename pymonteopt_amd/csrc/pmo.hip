@@ -639,18 +639,28 @@ int diff_launch(const pmo_volume* vol, const pmo_source* src, const pmo_run* run
     P.mismatch = reinterpret_cast<unsigned long long*>(diff->mismatch);
     P.work = reinterpret_cast<unsigned long long*>(workspace);
     if (mode == 1 && !P.mismatch) P.mismatch = P.work + 2;   // scratch counter in the workspace
+    // replay deposits through cube lines: a deposit's slot is 8 x cube + element, which must
+    // stay below 2^31 (else the per-voxel atomics)
+    P.nly = (uint32_t)((vol->ny + 1) / 2);
+    P.nlz = (uint32_t)((vol->nz + 1) / 2);
+    const unsigned long long n_cubes = (unsigned long long)((vol->nx + 1) / 2) * P.nly * P.nlz;
+    const bool lines = mode == 1 && diff->grad_lines != nullptr && n_cubes * 8ull < (1ull << 31);
+    P.lines = lines ? diff->grad_lines : nullptr;
+    P.lbuf_stride = (uint32_t)bs;
     const uint32_t njac = P.jac ? P.n_labels : 0u;
     P.lds_tab = 0;
     P.lds_scal = (uint32_t)align_up((size_t)(vol->n_labels + 1) * sizeof(pmo::LabelConst<float>), 16);
     P.lds_ev = P.lds_scal + (uint32_t)align_up(pmo::kBS_COUNT * sizeof(double), 16);
     P.lds_hist = P.lds_ev + (uint32_t)align_up(pmo::kBE_COUNT * sizeof(unsigned long long), 16);
     P.lds_jac = P.lds_hist + (uint32_t)align_up((size_t)P.nr * sizeof(double), 16);
-    P.lds_win = P.lds_jac + (uint32_t)align_up((size_t)P.nr * njac * 2u * sizeof(double), 16);
+    P.lds_lbuf = P.lds_jac + (uint32_t)align_up((size_t)P.nr * njac * 2u * sizeof(double), 16);
+    P.lds_stage = P.lds_lbuf + (lines ? (uint32_t)(8u * (uint32_t)bs * sizeof(double)) : 0u);
+    P.lds_win = P.lds_stage + (lines ? (uint32_t)((uint32_t)bs / 64u * 16u * sizeof(int32_t)) : 0u);
     if (P.lds_win > d.max_lds) return fail(PMO_ERR_UNSUPPORTED, "differentiable walk needs %u B of LDS", P.lds_win);
     // replay: LDS box of the gradient map around the launch voxel (as for the absorbed
     // tally), shrunk depth first to the LDS left
     if (mode == 1) {
-        const int* wdef = bs >= kWideBlock ? kWideWindow : kDefaultWindow;
+        const int* wdef = bs >= kWideBlock ? (lines ? kLinesWideWindow : kWideWindow) : kDefaultWindow;
         int wdim[3] = {wdef[0], wdef[1], wdef[2]};
         const int ext[3] = {vol->nx, vol->ny, vol->nz};
         const int ctr[3] = {src->ix, src->iy, src->iz};
@@ -678,7 +688,8 @@ int diff_launch(const pmo_volume* vol, const pmo_source* src, const pmo_run* run
     if (lds > d.max_lds) return fail(PMO_ERR_UNSUPPORTED, "differentiable walk needs %zu B of LDS", lds);
 
     int nb = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::diff_kernel<float>, bs, lds);
+    hipError_t e = lines ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::diff_kernel<float, true>, bs, lds)
+                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::diff_kernel<float, false>, bs, lds);
     if (e != hipSuccess) return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     if (nb < 1) nb = 1;
     unsigned long long grid = run->grid_blocks > 0 ? (unsigned long long)run->grid_blocks
@@ -699,9 +710,22 @@ int diff_launch(const pmo_volume* vol, const pmo_source* src, const pmo_run* run
     hipLaunchKernelGGL(pmo::zero_u64_kernel, dim3(1), dim3(64), 0, s, P.work, 4);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "zero workspace launch");
-    hipLaunchKernelGGL(pmo::diff_kernel<float>, dim3((unsigned)grid), dim3(bs), lds, s, P);
+    const unsigned cube_grid = (unsigned)((n_cubes + 255ull) / 256ull < 8192ull ? (n_cubes + 255ull) / 256ull : 8192ull);
+    if (lines) {
+        hipLaunchKernelGGL(pmo::zero_f64_kernel, dim3(cube_grid), dim3(256), 0, s, P.lines, n_cubes * 8ull);
+        e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "zero line scratch launch");
+    }
+    if (lines) hipLaunchKernelGGL((pmo::diff_kernel<float, true>), dim3((unsigned)grid), dim3(bs), lds, s, P);
+    else hipLaunchKernelGGL((pmo::diff_kernel<float, false>), dim3((unsigned)grid), dim3(bs), lds, s, P);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "differentiable walk launch");
+    if (lines) {
+        hipLaunchKernelGGL(pmo::lines_to_absorbed_kernel, dim3(cube_grid), dim3(256), 0, s, (const double*)P.lines,
+                           P.grad, P.nx, P.ny, P.nz, P.nly, P.nlz, n_cubes);
+        e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "line scratch transfer launch");
+    }
     return PMO_OK;
 }
 

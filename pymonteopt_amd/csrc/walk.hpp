@@ -341,7 +341,7 @@ __device__ __forceinline__ uint32_t line_slot(uint32_t e, uint32_t t, uint32_t s
 // add the non-zero ones to its cube's line with one atomic instruction, so each buffer costs
 // one 64 B request; the readers clear what they read. Wave-uniform: every lane of the wave
 // must call it together. The stage (16 words per wave) passes buffer owners and lines.
-__device__ __forceinline__ void flush_lines(const WalkParams& P, double* lbuf, int32_t* stage, bool want,
+__device__ __forceinline__ void flush_lines(double* lines, uint32_t stride, double* lbuf, int32_t* stage, bool want,
                                             int32_t& cur_line, unsigned long long lt_mask, uint32_t& n_req)
 {
     const uint32_t lane = threadIdx.x & 63u;
@@ -359,10 +359,10 @@ __device__ __forceinline__ void flush_lines(const WalkParams& P, double* lbuf, i
         if (slot < nf) {
             const uint32_t owner = (uint32_t)st[slot];
             const uint32_t line = (uint32_t)st[8u + slot];
-            double* q = lbuf + line_slot(j, owner, P.lbuf_stride);
+            double* q = lbuf + line_slot(j, owner, stride);
             const double v = *q;
             *q = 0.0;
-            if (v != 0.0) gmem_add(P.lines + (size_t)line * 8u + j, v);
+            if (v != 0.0) gmem_add(lines + (size_t)line * 8u + j, v);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -444,7 +444,7 @@ __device__ __forceinline__ void walk_body(const WalkParams& P)
     for (;;) {
         // ---------------------------------------------------- line buffers (all lanes here)
         if (use_lines) {
-            flush_lines(P, lbuf, stage, p_line >= 0 && cur_line >= 0, cur_line, lt_mask, n_glob);
+            flush_lines(P.lines, P.lbuf_stride, lbuf, stage, p_line >= 0 && cur_line >= 0, cur_line, lt_mask, n_glob);
             if (p_line >= 0) {
                 cur_line = p_line;
                 lds_add(lbuf + line_slot(p_e, threadIdx.x, P.lbuf_stride), p_dep);
@@ -809,13 +809,13 @@ __device__ __forceinline__ void walk_body(const WalkParams& P)
     // ---------------------------------------------------------------- block flush
     if (use_lines) {
         // the deposits still waiting, then every line buffer (all lanes are here)
-        flush_lines(P, lbuf, stage, p_line >= 0 && cur_line >= 0, cur_line, lt_mask, n_glob);
+        flush_lines(P.lines, P.lbuf_stride, lbuf, stage, p_line >= 0 && cur_line >= 0, cur_line, lt_mask, n_glob);
         if (p_line >= 0) {
             cur_line = p_line;
             lds_add(lbuf + line_slot(p_e, threadIdx.x, P.lbuf_stride), p_dep);
             p_line = -1;
         }
-        flush_lines(P, lbuf, stage, cur_line >= 0, cur_line, lt_mask, n_glob);
+        flush_lines(P.lines, P.lbuf_stride, lbuf, stage, cur_line >= 0, cur_line, lt_mask, n_glob);
     }
     if (P.absorbed) flush_deposit(P, win, dep_slot, dep_acc, n_glob);
     lds_add(&bscal[kBS_ABS], acc_abs);
@@ -907,8 +907,9 @@ __global__ void __launch_bounds__(256) zero_f64_kernel(double* p, unsigned long 
     for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) p[i] = 0.0;
 }
 
-// absorbed[x][y][z] += lines[cube of (x, y, z)][element]: the cube-major line sums into the
-// C-order map; one thread per cube, its 8 elements to the up-to-8 voxels inside the volume
+// map[x][y][z] += lines[cube of (x, y, z)][element]: the cube-major line sums into the C-order
+// map (the absorbed map, or the replay's gradient); one thread per cube, its 8 elements to the
+// up-to-8 voxels inside the volume
 __global__ void __launch_bounds__(256) lines_to_absorbed_kernel(const double* lines, double* absorbed, uint32_t nx,
                                                                 uint32_t ny, uint32_t nz, uint32_t nly, uint32_t nlz,
                                                                 unsigned long long n_cubes)

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), n
         assert n in _abi.SIGNATURES, n
-    assert L.pmo_abi_version() == _abi.ABI_VERSION == 7
+    assert L.pmo_abi_version() == _abi.ABI_VERSION == 8
     assert L.pmo_workspace_bytes() >= 8
 
 
@@ -43,6 +43,7 @@ int main(void) {
   O(pmo_run, packets_per_lane) O(pmo_tallies, exit_state)
   O(pmo_tallies, ring_w) O(pmo_tallies, line_z) O(pmo_tallies, fluence_rz) O(pmo_run, flags) O(pmo_run, window)
   O(pmo_tallies, exit_add) O(pmo_tallies, exit_p) O(pmo_tallies, absorbed_lines) O(pmo_run, refill)
+  O(pmo_diff, grad_lines)
   printf("lines_3_5_4 %llu\n", PMO_ABSORBED_LINES_DOUBLES(3, 5, 4));
   O(pmo_obd, stream) O(pmo_obd, side)
   printf("PMO_S_COUNT %d\nPMO_EV_COUNT %d\nPMO_OBD_STEP_FIELDS %d\n", PMO_S_COUNT, PMO_EV_COUNT, PMO_OBD_STEP_FIELDS);
@@ -84,6 +85,7 @@ def test_ctypes_layout_matches_header(tmp_path):
     assert int(out["pmo_tallies.exit_p"]) == _abi.PmoTallies.exit_p.offset
     assert int(out["pmo_tallies.absorbed_lines"]) == _abi.PmoTallies.absorbed_lines.offset
     assert int(out["pmo_run.refill"]) == _abi.PmoRun.refill.offset
+    assert int(out["pmo_diff.grad_lines"]) == _abi.PmoDiff.grad_lines.offset
     assert int(out["lines_3_5_4"]) == _abi.absorbed_lines_doubles(3, 5, 4) == 8 * 2 * 3 * 2
     assert int(out["PMO_S_COUNT"]) == _abi.PMO_S_COUNT
     assert int(out["pmo_obd.stream"]) == _abi.PmoObd.stream.offset
