@@ -214,10 +214,12 @@ def rd_of_absorption_field(field, dw: DiffWalk, seed=0, stream_id=0):
 
 
 def fit_layers(dw: DiffWalk, target_rd_r, g, mua0, musp0, iters=150, lr=0.05, seed0=1000, bins=None,
-               callback=None):
+               callback=None, lr_final=None, average=0):
     """Adam on log mu_a and log mu_s' of each label so that the walk's Rd(r) matches
     `target_rd_r` (relative squared error over `bins`); g stays fixed, mu_s = mu_s'/(1-g).
-    Each iteration walks with a fresh seed."""
+    Each iteration walks with a fresh seed, so the iterates carry that walk's noise: with
+    `lr_final` the step size decays geometrically from `lr` to it, and with `average` > 0
+    the result is the mean of the last `average` iterates (in log space)."""
     dev = dw.device
     g = torch.as_tensor(np.asarray(g, np.float64), device=dev)
     la = torch.log(torch.as_tensor(np.asarray(mua0, np.float64), device=dev)).requires_grad_(True)
@@ -225,7 +227,10 @@ def fit_layers(dw: DiffWalk, target_rd_r, g, mua0, musp0, iters=150, lr=0.05, se
     target = torch.as_tensor(np.asarray(target_rd_r, np.float64), device=dev)
     sel = torch.arange(target.numel(), device=dev) if bins is None else torch.as_tensor(bins, device=dev)
     opt = torch.optim.Adam([la, ls], lr=lr)
+    sched = (torch.optim.lr_scheduler.ExponentialLR(opt, gamma=(lr_final / lr) ** (1.0 / max(iters - 1, 1)))
+             if lr_final else None)
     hist = []
+    acc_a, acc_s, n_acc = torch.zeros_like(la), torch.zeros_like(ls), 0
     for it in range(iters):
         opt.zero_grad()
         mua = torch.exp(la)
@@ -234,7 +239,16 @@ def fit_layers(dw: DiffWalk, target_rd_r, g, mua0, musp0, iters=150, lr=0.05, se
         loss = (((rd[sel] - target[sel]) / target[sel]) ** 2).mean()
         loss.backward()
         opt.step()
+        if sched is not None:
+            sched.step()
+        if average and it >= iters - average:
+            acc_a += la.detach()
+            acc_s += ls.detach()
+            n_acc += 1
         hist.append(float(loss.detach()))
         if callback:
-            callback(it, float(loss), torch.exp(la).detach().cpu().numpy(), torch.exp(ls).detach().cpu().numpy())
+            callback(it, float(loss.detach()), torch.exp(la).detach().cpu().numpy(),
+                     torch.exp(ls).detach().cpu().numpy())
+    if n_acc:
+        return torch.exp(acc_a / n_acc).cpu().numpy(), torch.exp(acc_s / n_acc).cpu().numpy(), hist
     return torch.exp(la).detach().cpu().numpy(), torch.exp(ls).detach().cpu().numpy(), hist

@@ -250,9 +250,39 @@ def test_adjoint_cost_vs_forward(grid, skin):
         assert ta <= 3.0 * tf, (wl.name, ta, tf)
 
 
+def _log_param_sigma(dw, mua, musp, g, target, n_jac=5, n_var=24):
+    """Predicted standard error of the fitted log mu_a, log mu_s' of each layer from the
+    target's own noise (linearised least squares of the fit's relative-error loss): the
+    Jacobian of Rd(r) at the truth from n_jac x 1e7 packets of pMC sums, the per-bin
+    variance of a 1e7-packet Rd(r) from n_var 1e6-packet seeds."""
+    opt = dw.base_optics.copy()
+    opt[:3, 0] = mua
+    opt[:3, 1] = musp / (1 - g)
+    dw.set_optics(opt)
+    dw.n_packets = 10_000_000
+    Js = []
+    for s in range(n_jac):
+        f = dw.forward(seed=1000 + s, jac=True)
+        Js.append(np.concatenate([f["J_mua"].cpu().numpy()[:, :3] * mua[None, :],
+                                  f["J_mus"].cpu().numpy()[:, :3] * (musp / (1 - g))[None, :]], 1))
+    J = np.mean(Js, 0)
+    dw.n_packets = 1_000_000
+    var = np.array([dw.forward(seed=200 + k, jac=False)["Rd_r"].cpu().numpy() for k in range(n_var)]).var(0, ddof=1)
+    dw.set_optics(dw.base_optics)
+    W = 1.0 / np.asarray(target) ** 2
+    Ai = np.linalg.inv(J.T @ (W[:, None] * J))
+    cov = Ai @ (J.T @ ((W * (var / 10) * W)[:, None] * J)) @ Ai
+    return np.sqrt(np.diag(cov))
+
+
 def test_adam_recovers_layer_optics(skin):
     """Inverse problem: 3-layer (mu_a, mu_s') drawn within +-30% of the skin values,
-    target Rd(r) from a 1e7-packet walk, Adam from the nominal values."""
+    target Rd(r) from a 1e7-packet walk, Adam from the nominal values. Every one of the six
+    parameters must land within 4 predicted standard errors of its hidden value, the error
+    Rd(r) at 1e7 packets leaves it (the Fisher matrix of the fit's loss at the truth): that
+    is 0.5% and 1.5% for the epidermis and dermis mu_s', but 7-21% for the three mu_a and the
+    subcutis mu_s', which Rd(r) over 4 mm only weakly separates (epidermis and dermis mu_a
+    are anti-correlated at -0.99)."""
     from pymonteopt_amd.adjoint import DiffWalk, fit_layers
     dw = DiffWalk(skin.labels, skin.pitch, skin.optics, skin.source, n_packets=10_000_000, nr=40, dr=0.1)
     rng = np.random.default_rng(1234)
@@ -266,10 +296,12 @@ def test_adam_recovers_layer_optics(skin):
     opt[:3, 1] = musp_t / (1 - g)
     dw.set_optics(opt)
     target = dw.forward(seed=99, jac=False)["Rd_r"].cpu().numpy()
+    dw.set_optics(dw.base_optics)
+    sigma = _log_param_sigma(dw, mua_t, musp_t, g, target)
+    assert np.all(sigma[3:5] < 0.03)              # the two well-determined mu_s'
     dw.n_packets = 2_000_000
-    mua_f, musp_f, hist = fit_layers(dw, target, g, mua0, musp0, iters=200, lr=0.04)
+    mua_f, musp_f, hist = fit_layers(dw, target, g, mua0, musp0, iters=600, lr=0.04, lr_final=0.004, average=100)
     assert hist[-1] < 0.05 * hist[0]
-    # the well-determined parameters of Rd(r): epidermis and dermis mu_s', subcutis mu_a
-    assert abs(musp_f[0] / musp_t[0] - 1) < 0.1
-    assert abs(musp_f[1] / musp_t[1] - 1) < 0.1
-    assert abs(mua_f[2] / mua_t[2] - 1) < 0.15
+    err = np.log(np.concatenate([mua_f / mua_t, musp_f / musp_t]))
+    assert np.all(np.abs(err) < 4 * sigma), (err, sigma)
+    assert abs(err[3]) < 0.05 and abs(err[4]) < 0.05
