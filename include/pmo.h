@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define PMO_ABI_VERSION 8
+#define PMO_ABI_VERSION 9
 
 /* ---- error codes ---------------------------------------------------------------- */
 #define PMO_OK 0
@@ -335,7 +335,9 @@ int pmo_walk_adjoint(const pmo_volume* vol, const pmo_source* src, const pmo_run
  * PMO_PROBE_F64_PER_XCD, buf holds 8 copies of n doubles and each workgroup adds into the
  * copy of the XCD it runs on (PMO_PROBE_F64_PER_XCD_WG: the same at workgroup scope);
  * with PMO_PROBE_F64_LINES each wave adds to 64 consecutive doubles (eight 64 B lines) of a
- * pseudo-random 512 B block of buf[n], n >= 64. pmo_probe_valu runs blocks*256 lanes that
+ * pseudo-random 512 B block of buf[n], n >= 64; with PMO_PROBE_F64_BLOCKS_32B / _64B / _128B
+ * the lanes in groups of 4 / 8 / 16 each add to the consecutive doubles of their own
+ * pseudo-random 32 / 64 / 128 B-aligned block (n >= 16). pmo_probe_valu runs blocks*256 lanes that
  * each issue iters x 32 independent FP32 FMAs and write one float to out[blocks*256]. */
 enum {
     PMO_PROBE_F64 = 0,
@@ -344,7 +346,10 @@ enum {
     PMO_PROBE_U32 = 3,
     PMO_PROBE_F64_PER_XCD = 4,
     PMO_PROBE_F64_PER_XCD_WG = 5,
-    PMO_PROBE_F64_LINES = 6
+    PMO_PROBE_F64_LINES = 6,
+    PMO_PROBE_F64_BLOCKS_32B = 7,
+    PMO_PROBE_F64_BLOCKS_64B = 8,
+    PMO_PROBE_F64_BLOCKS_128B = 9
 };
 int pmo_probe_atomics(void* buf, int64_t n, int32_t blocks, int32_t iters, int32_t kind, void* stream);
 int pmo_probe_valu(float* out, int32_t blocks, int32_t iters, void* stream);

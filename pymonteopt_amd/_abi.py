@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpmo.so")
 
-ABI_VERSION = 8          # PMO_ABI_VERSION of include/pmo.h
+ABI_VERSION = 9          # PMO_ABI_VERSION of include/pmo.h
 
 PMO_OK = 0
 PMO_ERR_INVALID = -1
@@ -32,6 +32,7 @@ PMO_SKIP_MAX_CAP = 127
 
 PMO_PROBE_F64, PMO_PROBE_F32, PMO_PROBE_U64, PMO_PROBE_U32, PMO_PROBE_F64_PER_XCD = 0, 1, 2, 3, 4
 PMO_PROBE_F64_PER_XCD_WG, PMO_PROBE_F64_LINES = 5, 6
+PMO_PROBE_F64_BLOCKS_32B, PMO_PROBE_F64_BLOCKS_64B, PMO_PROBE_F64_BLOCKS_128B = 7, 8, 9
 
 SCALAR_NAMES = ["Rd", "Tt", "other", "A", "kill", "gain", "w0",
                 "face_x-", "face_x+", "face_y-", "face_y+", "face_z-", "face_z+"]

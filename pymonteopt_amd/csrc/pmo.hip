@@ -534,9 +534,14 @@ int pmo_probe_atomics(void* buf, int64_t n, int32_t blocks, int32_t iters, int32
                            iters, 1);
         break;
     case PMO_PROBE_F64_LINES:
-        if (m < 64) return fail(PMO_ERR_INVALID, "the line probe needs n >= 64");
-        hipLaunchKernelGGL(pmo::probe_atomics_lines_kernel, dim3(blocks), dim3(256), 0, s, (double*)buf, m, iters);
+    case PMO_PROBE_F64_BLOCKS_32B:
+    case PMO_PROBE_F64_BLOCKS_64B:
+    case PMO_PROBE_F64_BLOCKS_128B: {
+        const uint32_t lg = kind == PMO_PROBE_F64_LINES ? 6u : (uint32_t)(kind - PMO_PROBE_F64_BLOCKS_32B) + 2u;
+        if (m < (1u << lg)) return fail(PMO_ERR_INVALID, "this probe needs n >= %u", 1u << lg);
+        hipLaunchKernelGGL(pmo::probe_atomics_lines_kernel, dim3(blocks), dim3(256), 0, s, (double*)buf, m, iters, lg);
         break;
+    }
     default:
         return fail(PMO_ERR_INVALID, "unknown probe kind");
     }

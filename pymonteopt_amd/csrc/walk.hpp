@@ -949,16 +949,18 @@ __global__ void probe_atomics_kernel(T* buf, uint32_t n, int iters, int per_xcd)
     }
 }
 
-// The same adds with each wave's 64 lanes on 64 consecutive doubles (eight 64 B lines) of a
-// pseudo-random 512 B block of buf[n] (n >= 64): the rate per element when every request
-// carries eight adds to one line.
-__global__ void probe_atomics_lines_kernel(double* buf, uint32_t n, int iters)
+// The same adds with the lanes in groups of 2^lg (lg = 2..6): each group adds to the 2^lg
+// consecutive doubles of its own pseudo-random, 2^lg-aligned block of buf[n] (n >= 2^lg),
+// one double per lane. lg = 6: a wave on eight 64 B lines of one 512 B block; lg = 2, 3, 4:
+// a wave on 16 random 32 B, 8 random 64 B or 4 random 128 B blocks. Every address is below
+// (n >> lg) << lg <= n.
+__global__ void probe_atomics_lines_kernel(double* buf, uint32_t n, int iters, uint32_t lg)
 {
-    uint32_t x = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 2654435761u ^ 2463534242u;
-    const uint32_t lane = threadIdx.x & 63u, nb = n >> 6;
+    uint32_t x = ((blockIdx.x * blockDim.x + threadIdx.x) >> lg) * 2654435761u ^ 2463534242u;
+    const uint32_t lane = threadIdx.x & ((1u << lg) - 1u), nb = n >> lg;
     for (int i = 0; i < iters; ++i) {
         x ^= x << 13; x ^= x >> 17; x ^= x << 5;
-        __hip_atomic_fetch_add(buf + (size_t)(x % nb) * 64u + lane, 1.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(buf + ((size_t)(x % nb) << lg) + lane, 1.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 

@@ -73,6 +73,16 @@ def peaks():
             out[f"f64_wave_on_8_lines_atomics_per_s_{mib}MiB"] = count / t
             torch.cuda.synchronize()
             out[f"f64_wave_on_8_lines_sum_ok_{mib}MiB"] = bool(float(buf.sum()) == 2.0 * count)
+            # lanes in groups of 4 / 8 / 16 on their own random 32 / 64 / 128 B blocks: which
+            # block size one atomic request covers
+            for tag, kind in (("32B", _abi.PMO_PROBE_F64_BLOCKS_32B), ("64B", _abi.PMO_PROBE_F64_BLOCKS_64B),
+                              ("128B", _abi.PMO_PROBE_F64_BLOCKS_128B)):
+                buf.zero_()
+                t = timed(lambda: _abi.check(L.pmo_probe_atomics(C.c_void_p(buf.data_ptr()), n, blocks, iters,
+                                                                 kind, s)))
+                out[f"f64_lanes_on_random_{tag}_blocks_atomics_per_s_{mib}MiB"] = count / t
+                torch.cuda.synchronize()
+                out[f"f64_lanes_on_random_{tag}_blocks_sum_ok_{mib}MiB"] = bool(float(buf.sum()) == 2.0 * count)
     o = torch.empty(2048 * 256, dtype=torch.float32, device="cuda:0")
     t = timed(lambda: _abi.check(L.pmo_probe_valu(C.c_void_p(o.data_ptr()), 2048, 8192, s)))
     out["valu_fp32_fma_lane_ops_per_s"] = 2048 * 256.0 * 8192 * 32 / t
