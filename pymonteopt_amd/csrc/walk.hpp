@@ -26,12 +26,13 @@ template <> struct Mth<float> {
     static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
     static __device__ __forceinline__ float rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
     static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-    static __device__ __forceinline__ float neglog(float u) { return -__logf(u); }
+    // -ln u for u in [2^-24, 1] (never denormal): the hardware log2 times -ln 2
+    static __device__ __forceinline__ float neglog(float u) { return __builtin_amdgcn_logf(u) * -0.693147180559945309f; }
+    // sin and cos of 2 pi u for u in [0, 1]: the hardware sine and cosine take revolutions
     static __device__ __forceinline__ void sincos2pi(float u, float* s, float* c)
     {
-        const float a = 6.28318530717958647f * u;
-        *s = __sinf(a);
-        *c = __cosf(a);
+        *s = __builtin_amdgcn_sinf(u);
+        *c = __builtin_amdgcn_cosf(u);
     }
     static __device__ __forceinline__ float fabs(float x) { return __builtin_fabsf(x); }
     static __device__ __forceinline__ float copysign(float a, float b) { return __builtin_copysignf(a, b); }

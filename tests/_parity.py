@@ -55,3 +55,28 @@ def packet_events(r):
     """The per-packet event counts of a run: every counter except the atomic-request count,
     which depends on how packets shared lanes (and their deposit buffers) in turn."""
     return {k: v for k, v in r["events"].items() if k != "global_deposits"}
+
+
+def binned_equal_up_to_edges(x, w, edges, got, eps):
+    """The device's per-bin sums `got` against host binning of the dumped values x (bins
+    (edges[i], edges[i+1]]): equal except for the weight of packets within eps of an edge,
+    which FP32 rounding on the device may put on either side. Returns the largest excess of
+    |difference| over that slack, relative to the largest bin (<= 0 when they agree)."""
+    x = np.asarray(x, np.float64)
+    w = np.asarray(w, np.float64)
+    edges = np.asarray(edges, np.float64)
+    idx = np.searchsorted(edges, x, side="left") - 1
+    ok = (idx >= 0) & (idx < edges.size - 1) & (x > edges[0])
+    want = np.bincount(idx[ok], weights=w[ok], minlength=edges.size - 1)
+    j = np.clip(np.searchsorted(edges, x), 0, edges.size - 1)
+    near = (np.abs(x - edges[j]) < eps) | (np.abs(x - edges[np.maximum(j - 1, 0)]) < eps)
+    slack = np.zeros(edges.size - 1)
+    # a near-edge packet may move into either bin beside its edge
+    ne = np.argmin(np.abs(x[near][:, None] - edges[None, :]), axis=1)
+    for k, wk in zip(ne, w[near]):
+        if k - 1 >= 0:
+            slack[k - 1] += wk
+        if k < edges.size - 1:
+            slack[k] += wk
+    scale = max(float(np.max(np.abs(want))), 1e-300)
+    return float(np.max(np.abs(np.asarray(got, np.float64) - want) - slack - 1e-9 * scale) / scale)

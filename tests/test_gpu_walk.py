@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from _parity import binned_z, gpu_run, ledger_error, oracle_run, packet_events, zscore
+from _parity import binned_equal_up_to_edges, binned_z, gpu_run, ledger_error, oracle_run, packet_events, zscore
 
 pytestmark = pytest.mark.gpu
 
@@ -117,16 +117,20 @@ def test_exit_state_dump_matches_tallies(slab):
     rd = v[2] < 0
     assert w[rd].sum(dtype=np.float64) == pytest.approx(r["scalars"]["Rd"], rel=1e-6)
     assert w[v[2] > 0].sum(dtype=np.float64) == pytest.approx(r["scalars"]["Tt"], rel=1e-6)
+    # host binning of the dump (metrics.py, float64) against the device's FP32 binning: equal
+    # but for packets within rounding distance of a bin edge, which may fall either side
+    pr = p.astype(np.float64)
+    r_xy = np.sqrt(pr[0, rd] ** 2 + pr[1, rd] ** 2)
+    assert binned_equal_up_to_edges(r_xy, w[rd], np.arange(51) * 0.1, r["rd_r"], 1e-5) <= 0
+    ang = np.arccos(np.abs(v[2, rd].astype(np.float64)))
+    assert binned_equal_up_to_edges(ang, w[rd], np.arange(31) * (np.pi / 60), r["rd_a"], 1e-5) <= 0
+    side = (v[0] < 0) & (np.abs(p[1]) < 5.0)
+    assert binned_equal_up_to_edges(pr[2, side], w[side], np.arange(41) * 0.125, r["line_z"], 1e-5) <= 0
+    # and radialDistance normalises its own binning of the dump by the annulus areas
     _, rdr = metrics.radialDistance(p[:, rd], w[rd], 50, 0.1, N)
     da = np.array([2 * np.pi * (i + 0.5) * 0.1 ** 2 for i in range(50)])
-    assert np.allclose(rdr, r["rd_r"] / (da * N), rtol=1e-5, atol=1e-12)
-    _, rda = metrics.angularyResolved(v[:, rd], w[rd], 30, N)
-    d = np.pi / 60
-    alpha = np.array([(i + 0.5) * d for i in range(31)])
-    do = 4 * np.pi * np.sin(alpha) * np.sin(d / 2)
-    assert np.allclose(rda, r["rd_a"] / (do[:-1] * N), rtol=1e-4, atol=1e-9)
-    _, line = metrics.lineDistance(p[:, v[0] < 0], w[v[0] < 0], 40, 0.125, N, y_range=5.0)
-    assert np.allclose(line, r["line_z"] / (0.125 * 5.0 * 2 * N), rtol=1e-5, atol=1e-12)
+    r32 = np.sqrt(p[0, rd] ** 2 + p[1, rd] ** 2)
+    assert np.allclose(rdr * da * N, metrics._binned(r32, w[rd], np.arange(51) * 0.1), rtol=1e-12, atol=1e-15)
 
 
 def test_known_answers_on_gpu(torch_cuda):
