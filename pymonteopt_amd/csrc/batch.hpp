@@ -158,6 +158,10 @@ plate_batch_kernel(const BatchParams P)
                     ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
                     w = (R)I.w0;
                     acc_w0 += (double)I.w0;
+                    // absorbed weight telescoped over the packet: each drop is exactly
+                    // (double)w - (double)w_new, so their sum is the launch weight, minus
+                    // the final weight, plus the roulette gains (added at the end)
+                    acc_abs += (double)w;
                     lab = plate_label(G, ax, ay, az);
                     if constexpr (LAYER) {
                         // plate coordinates of the launch point (the exit encoding of
@@ -301,16 +305,16 @@ plate_batch_kernel(const BatchParams P)
             } else {
                 const R sm = M::div(st, L.mt);
                 px += vx * sm; py += vy * sm; pz += vz * sm;
-                const R wn = w - w * L.ra;
-                acc_abs += (double)w - (double)wn;
-                w = wn;
+                w = w - w * L.ra;
                 if (w <= (R)P.wth) {
                     atomicAdd(&bev[kBE_ROULETTE], 1ull);
                     if ((R)P.inv_m < u_roul) {
                         killed = true;
                     } else {
                         const R wm = w * (R)P.roulette_m;
-                        gmem_add(P.scalars + (size_t)inst * kBS_COUNT + kBS_GAIN, (double)wm - (double)w);
+                        const double gain = (double)wm - (double)w;
+                        gmem_add(P.scalars + (size_t)inst * kBS_COUNT + kBS_GAIN, gain);
+                        acc_abs += gain;
                         w = wm;
                     }
                 }
@@ -329,6 +333,7 @@ plate_batch_kernel(const BatchParams P)
         }
 
         if (ended || killed) {
+            acc_abs -= (double)w;
             atomicAdd(&bev[kBE_INTERACT], (unsigned long long)inter);
             atomicAdd(&bev[kBE_FRESNEL], (unsigned long long)fres);
             atomicAdd(&bev[kBE_PACKETS], 1ull);

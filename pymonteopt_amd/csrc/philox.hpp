@@ -38,10 +38,13 @@ __host__ __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, ui
 
 // uniform on (0, 1]: top 24 bits plus one, times 2^-24; exact in float and double, never 0
 // so -log(u) is finite (the reference's curand_uniform range, _cukernel.py:75).
+// Written as k 2^-24 + 2^-24 with k = x >> 8 < 2^24: both steps are exact, so the value is
+// (k + 1) 2^-24 whether or not they are fused, in one convert and one FMA.
 template <typename R>
 __host__ __device__ __forceinline__ R u01(uint32_t x)
 {
-    return (R)((x >> 8) + 1u) * (R)(1.0 / 16777216.0);
+    const R s = (R)(1.0 / 16777216.0);
+    return (R)(x >> 8) * s + s;
 }
 
 }  // namespace pmo

@@ -40,6 +40,8 @@ template <> struct Mth<float> {
     static __device__ __forceinline__ float fmin(float a, float b) { return __builtin_fminf(a, b); }
     static __device__ __forceinline__ float floor(float x) { return __builtin_floorf(x); }
     static __device__ __forceinline__ float rint(float x) { return __builtin_rintf(x); }
+    // x clamped to [-1, 1] (x is never NaN here): one median-of-three
+    static __device__ __forceinline__ float clamp1(float x) { return __builtin_amdgcn_fmed3f(x, -1.0f, 1.0f); }
 };
 
 template <> struct Mth<double> {
@@ -58,6 +60,7 @@ template <> struct Mth<double> {
     static __device__ __forceinline__ double fmin(double a, double b) { return __builtin_fmin(a, b); }
     static __device__ __forceinline__ double floor(double x) { return __builtin_floor(x); }
     static __device__ __forceinline__ double rint(double x) { return __builtin_rint(x); }
+    static __device__ __forceinline__ double clamp1(double x) { return x < -1.0 ? -1.0 : (x > 1.0 ? 1.0 : x); }
 };
 
 // per-label constants in LDS, derived once per block from (mu_a, mu_s, g, n)
@@ -251,7 +254,7 @@ __device__ __forceinline__ R hg_cos(const LabelConst<R>& L, R u)
     if (L.g != (R)0) {
         const R t = L.hg2 * M::rcp(L.hg3 + L.hg4 * u);
         const R c = (L.hg1 - t * t) * L.hginv;
-        return c < (R)-1 ? (R)-1 : (c > (R)1 ? (R)1 : c);
+        return M::clamp1(c);
     }
     return (R)2 * u - (R)1;
 }
