@@ -47,6 +47,9 @@ int main(void) {
   printf("lines_3_5_4 %llu\n", PMO_ABSORBED_LINES_DOUBLES(3, 5, 4));
   O(pmo_obd, stream) O(pmo_obd, side)
   printf("PMO_S_COUNT %d\nPMO_EV_COUNT %d\nPMO_OBD_STEP_FIELDS %d\n", PMO_S_COUNT, PMO_EV_COUNT, PMO_OBD_STEP_FIELDS);
+  printf("probes %d,%d,%d,%d,%d,%d,%d,%d,%d,%d\n", PMO_PROBE_F64, PMO_PROBE_F32, PMO_PROBE_U64, PMO_PROBE_U32,
+         PMO_PROBE_F64_PER_XCD, PMO_PROBE_F64_PER_XCD_WG, PMO_PROBE_F64_LINES, PMO_PROBE_F64_BLOCKS_32B,
+         PMO_PROBE_F64_BLOCKS_64B, PMO_PROBE_F64_BLOCKS_128B);
   return 0;
 }
 """
@@ -88,6 +91,10 @@ def test_ctypes_layout_matches_header(tmp_path):
     assert int(out["pmo_diff.grad_lines"]) == _abi.PmoDiff.grad_lines.offset
     assert int(out["lines_3_5_4"]) == _abi.absorbed_lines_doubles(3, 5, 4) == 8 * 2 * 3 * 2
     assert int(out["PMO_S_COUNT"]) == _abi.PMO_S_COUNT
+    assert [int(x) for x in out["probes"].split(",")] == [
+        _abi.PMO_PROBE_F64, _abi.PMO_PROBE_F32, _abi.PMO_PROBE_U64, _abi.PMO_PROBE_U32, _abi.PMO_PROBE_F64_PER_XCD,
+        _abi.PMO_PROBE_F64_PER_XCD_WG, _abi.PMO_PROBE_F64_LINES, _abi.PMO_PROBE_F64_BLOCKS_32B,
+        _abi.PMO_PROBE_F64_BLOCKS_64B, _abi.PMO_PROBE_F64_BLOCKS_128B]
     assert int(out["pmo_obd.stream"]) == _abi.PmoObd.stream.offset
     assert int(out["pmo_obd.side"]) == _abi.PmoObd.side.offset
     assert int(out["PMO_OBD_STEP_FIELDS"]) == _abi.PMO_OBD_STEP_FIELDS
