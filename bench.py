@@ -220,12 +220,16 @@ def adjoint_leg(n_packets=10_000_000):
         _, t_plain = timed(lambda: wk.walk(wl.source, n_packets, 3, tal))
         _, t_plain_v = timed(lambda: wkv.walk(wl.source, n_packets, 3, tal))
         if name == "skin":
-            fwd, t_fwd = timed(lambda: dw.forward(seed=3, jac=True))
+            # rd_of_label_optics: one forward with Jacobians, crossing same-label cubes in one
+            # step (no replay follows); the per-voxel replay needs a voxel-stepping forward
+            _, t_fwd = timed(lambda: dw.forward(seed=3, jac=True, replay=False))
+            fwd, t_fwd_v = timed(lambda: dw.forward(seed=3, jac=True))
             _, t_adj = timed(lambda: dw.adjoint(fwd, dw.area.clone()))
             t_grad = t_fwd
             rec = {"gradient": "per-label mu_a, mu_s: forward with Jacobians", "forward_with_jacobian_s": t_fwd,
+                   "forward_with_jacobian_voxel_stepping_s": t_fwd_v,
                    "per_voxel_replay_s": t_adj, "per_voxel_replay_modes_s": replay_modes(fwd),
-                   "per_voxel_gradient_cost_vs_plain": (t_fwd + t_adj) / t_plain}
+                   "per_voxel_gradient_cost_vs_plain": (t_fwd_v + t_adj) / t_plain}
         else:
             mua = torch.tensor(wl.optics[:, 0], device="cuda:0")
             field = mua[torch.as_tensor(np.minimum(wl.labels, wl.n_labels), device="cuda:0").long()].float()

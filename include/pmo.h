@@ -109,8 +109,8 @@ typedef struct pmo_volume {
     const uint8_t* skip;   /* device [nx][ny][nz], pmo_skip_radius of these labels, or NULL.
                               With it pmo_walk crosses the faces inside same-label regions in
                               one step (the same draws; merged arithmetic, rounded once); NULL
-                              steps voxel by voxel as the reference does. Other entry points
-                              ignore it.                                                       */
+                              steps voxel by voxel as the reference does. pmo_walk_forward_diff
+                              uses it without mua_field; other entry points ignore it.        */
 } pmo_volume;
 
 typedef struct pmo_source {
@@ -316,8 +316,12 @@ int pmo_philox(const uint32_t* ctr, uint32_t key0, uint32_t key1, uint32_t* out,
 /* Forward pass of the differentiable walk: the walk of pmo_walk restricted to the scalar
  * tallies (into `scalars` [PMO_S_COUNT], `events` [PMO_EV_COUNT] or NULL) and the detected
  * signal Rd(r), plus the perturbation-MC sums of pmo_diff.jac and the per-packet exit
- * record. No reference counterpart: the reference estimates optical properties by a
- * parameter sweep (.ipynb_checkpoints/si_in_光学特性推定-checkpoint.ipynb:13945-13967). */
+ * record. With vol->skip and no mua_field it crosses the faces inside same-label cubes in one
+ * step, as pmo_walk does (the same draws; the per-label sums change by rounding only); its
+ * exit record is then not the voxel-stepping replay's and must not be passed to
+ * pmo_walk_adjoint (which would count mismatches). No reference counterpart: the reference
+ * estimates optical properties by a parameter sweep
+ * (.ipynb_checkpoints/si_in_光学特性推定-checkpoint.ipynb:13945-13967). */
 int pmo_walk_forward_diff(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, double* scalars,
                           uint64_t* events, const pmo_diff* diff, void* workspace, void* stream);
 

@@ -36,8 +36,11 @@ class DiffWalk:
     detected in `nr` bins of `dr` mm."""
 
     def __init__(self, labels, pitch, optics, source: Source, n_packets=1e7, nr=40, dr=0.1, device="cuda:0",
-                 lines="auto"):
-        self.walker = Walker(labels, pitch, optics, device=device, skip=False)   # replay steps voxel by voxel
+                 lines="auto", skip=None):
+        # skip: forward passes that need no replay (the per-label Jacobians) cross same-label
+        # cubes in one step where the Walker finds that it pays (None) or always (True); the
+        # replay, and every forward it may replay, steps voxel by voxel
+        self.walker = Walker(labels, pitch, optics, device=device, skip=skip)
         self.device = self.walker.device
         self.base_optics = np.array(optics, np.float32)
         self.n_labels = self.base_optics.shape[0] - 1
@@ -64,8 +67,8 @@ class DiffWalk:
     def shape(self):
         return self.walker.shape
 
-    def _structs(self, seed, stream_id):
-        v = self.walker.volume_struct()
+    def _structs(self, seed, stream_id, skip=False):
+        v = self.walker.volume_struct(skip=skip)
         s = _abi.PmoSource()
         src = self.source
         s.kind = int(src.kind)
@@ -84,9 +87,11 @@ class DiffWalk:
     def set_optics(self, optics):
         self.walker.set_optics(np.asarray(optics, np.float32))
 
-    def forward(self, seed=0, stream_id=0, mua_field=None, jac=True):
+    def forward(self, seed=0, stream_id=0, mua_field=None, jac=True, replay=True):
         """One forward pass. Returns Rd(r) per unit area and packet, the Jacobians
-        dRd(r)/dmu_a and dRd(r)/dmu_s per label (when jac), and the exit record."""
+        dRd(r)/dmu_a and dRd(r)/dmu_s per label (when jac), and the exit record. replay=False
+        (without mua_field) lets it cross same-label cubes in one step: the same draws and
+        per-label sums up to rounding, faster, but its record cannot be replayed."""
         if jac and self.n_labels > 4:
             raise ValueError("per-label Jacobians support at most 4 labels")
         f64 = dict(dtype=torch.float64, device=self.device)
@@ -100,7 +105,8 @@ class DiffWalk:
             if tuple(field.shape) != tuple(self.shape):
                 raise ValueError(f"mua_field must have the padded volume shape {self.shape}")
         d = PmoDiff(_p(field), self.nr, self.dr, _p(rd), _p(J), _p(rec), None, None, None, None)
-        v, s, r = self._structs(seed, stream_id)
+        skip = not replay and field is None and self.walker.skip
+        v, s, r = self._structs(seed, stream_id, skip=skip)
         with torch.cuda.device(self.device):
             st = self.walker.launch_stream()
             rc = self.lib.pmo_walk_forward_diff(C.byref(v), C.byref(s), C.byref(r), _p(scal), None, C.byref(d),
@@ -110,7 +116,7 @@ class DiffWalk:
         norm = self.area * self.n_packets
         # the optics table this pass walked with: the replay must use the same one
         out = {"Rd_r": rd / norm, "scalars": scal, "exit_rec": rec, "field": field, "seed": seed,
-               "stream": stream_id, "optics": self.walker.optics.clone()}
+               "stream": stream_id, "optics": self.walker.optics.clone(), "replayable": not skip}
         if jac:
             mus = torch.as_tensor(self.walker.optics[: self.n_labels, 1].to(torch.float64))
             wl, wn = J[..., 0], J[..., 1]
@@ -124,6 +130,9 @@ class DiffWalk:
         if the walker's table changed since (another forward on this DiffWalk), and raises
         if any replayed packet left differently from its recorded exit: the gradient would
         then not be that of the forward pass."""
+        if not fwd.get("replayable", True):
+            raise ValueError("this forward pass crossed same-label cubes in one step (forward(replay=False)); "
+                             "run forward(replay=True) for a record the adjoint can replay")
         if self.lines == "auto":
             self._line_scratch()
             times = {}
@@ -178,7 +187,8 @@ class _LabelOptics(torch.autograd.Function):
         opt[:L, 0] = mua.detach().cpu().numpy()
         opt[:L, 1] = mus.detach().cpu().numpy()
         dw.set_optics(opt)
-        out = dw.forward(seed=seed, stream_id=stream_id, jac=True)
+        # the backward is J^T g: no replay, so the forward may cross same-label cubes in one step
+        out = dw.forward(seed=seed, stream_id=stream_id, jac=True, replay=False)
         ctx.save_for_backward(out["J_mua"], out["J_mus"])
         return out["Rd_r"].to(mua.dtype)
 

@@ -28,6 +28,7 @@ constexpr int kMaxJacLabels = 4;
 
 struct DiffParams {
     const uint8_t* labels;
+    const uint8_t* skip;           // forward (SKIP): uniform-cube radius per voxel (pmo_skip_radius)
     const float4* optics;
     const float* mua_field;        // per-voxel mu_a (padded index) or null
     uint32_t nx, ny, nz, n_labels;
@@ -61,8 +62,11 @@ struct DiffParams {
 };
 
 // LINES: the replay with line buffers, an instantiation of its own so that the forward keeps
-// its register budget
-template <typename R, bool LINES>
+// its register budget. SKIP: a forward without the per-voxel field that crosses the faces
+// inside same-label cubes in one step, as the plain walk does (the same draws, merged
+// arithmetic): the per-label sums need no voxel-by-voxel path, but its exit record is not
+// the voxel-stepping replay's, so it is never replayed.
+template <typename R, bool LINES, bool SKIP>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) diff_kernel(const DiffParams P)
 {
     using M = Mth<R>;
@@ -94,6 +98,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
     const R half = (R)P.pitch * (R)0.5;
     const R pitch = (R)P.pitch;
+    const R ipitch = (R)1 / pitch;
     const bool normal_limit = (P.flags & 2u) != 0u;
     const bool field = P.mua_field != nullptr;
 
@@ -103,6 +108,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
     R fb0 = 0, fb1 = 0, fb2 = 0;
     R L0 = 0, L1 = 0, L2 = 0, L3 = 0, N0 = 0, N1 = 0, N2 = 0, N3 = 0;   // per-label path stats
     int32_t ax = 0, ay = 0, az = 0;
+    int32_t rad = 0;                 // SKIP: uniform-cube radius around the voxel, a lower bound
     uint32_t lab = 0, inter = 0, fres = 0, nev = 0;
     unsigned long long pid = 0;
     double acc_abs = 0.0;
@@ -233,6 +239,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
                         w = (R)P.w0;
                         const uint32_t vi = ((uint32_t)ax * P.ny + (uint32_t)ay) * P.nz + (uint32_t)az;
                         const uint32_t l0 = P.labels[vi];
+                        if (SKIP) rad = P.skip[vi];
                         lab = l0 < end ? l0 : end;
                         ni = tab[lab].n;
                         if (lab != end) voxel_optics(lab, ax, ay, az);
@@ -261,10 +268,12 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
         bool want_int = false;
         const bool walking = alive && lab != end;
         if (walking) {
-            // only components with |v_i| > 0 take part (_kernel.py:65), as in the reference
-            const R d0 = vx != (R)0 ? (half - (vx > (R)0 ? px : -px)) * ivx : (R)INFINITY;
-            const R d1 = vy != (R)0 ? (half - (vy > (R)0 ? py : -py)) * ivy : (R)INFINITY;
-            const R d2 = vz != (R)0 ? (half - (vz > (R)0 ? pz : -pz)) * ivz : (R)INFINITY;
+            // only components with |v_i| > 0 take part (_kernel.py:65), as in the reference;
+            // with SKIP, the exit face of the uniform cube around the voxel
+            const R ext = SKIP ? (R)rad * pitch + half : half;
+            const R d0 = vx != (R)0 ? (ext - (vx > (R)0 ? px : -px)) * ivx : (R)INFINITY;
+            const R d1 = vy != (R)0 ? (ext - (vy > (R)0 ? py : -py)) * ivy : (R)INFINITY;
+            const R d2 = vz != (R)0 ? (ext - (vz > (R)0 ? pz : -pz)) * ivz : (R)INFINITY;
             if (d0 < db) { db = d0; k = 0; }
             if (d1 < db) { db = d1; k = 1; }
             if (d2 < db) { db = d2; k = 2; }
@@ -281,7 +290,24 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
             ended = true;
         } else if (want_int == run_int) {
             ++nev;
-            if (!want_int) {
+            if (!want_int && SKIP && rad > 0) {
+                // to the uniform cube's face (as in the plain walk): one label throughout
+                px += vx * db; py += vy * db; pz += vz * db;
+                if (njac) add_path(lab, db, (R)0);
+                const R vk = k == 0 ? vx : (k == 1 ? vy : vz);
+                st -= smt;
+                const int s = vk > (R)0 ? 1 : -1;
+                int ox = cube_relocate<R>(px, pitch, ipitch, half, rad);
+                int oy = cube_relocate<R>(py, pitch, ipitch, half, rad);
+                int oz = cube_relocate<R>(pz, pitch, ipitch, half, rad);
+                const R face = M::copysign(half, vk);
+                if (k == 0) { ox = s * rad; px = face; }
+                else if (k == 1) { oy = s * rad; py = face; }
+                else { oz = s * rad; pz = face; }
+                ax = clamp_index(ax + ox, P.nx); ay = clamp_index(ay + oy, P.ny); az = clamp_index(az + oz, P.nz);
+                n_cross += (uint32_t)(abs(ox) + abs(oy) + abs(oz));
+                rad = 0;            // on the cube's face: the next step crosses it voxel by voxel
+            } else if (!want_int) {
                 ++n_cross;
                 px += vx * db; py += vy * db; pz += vz * db;
                 if (replay) grad_segment(db);
@@ -295,9 +321,11 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
                 const int32_t by = ay + (k == 1 ? s : 0);
                 const int32_t bz = az + (k == 2 ? s : 0);
                 uint32_t nxt = end;
+                int32_t nrad = 0;
                 if ((uint32_t)bx < P.nx && (uint32_t)by < P.ny && (uint32_t)bz < P.nz) {
                     const uint32_t nvi = (__umul24((uint32_t)bx, P.ny) + (uint32_t)by) * P.nz + (uint32_t)bz;
                     nxt = P.labels[nvi];
+                    if (SKIP) nrad = P.skip[nvi];
                     nxt = nxt < end ? nxt : end;
                 }
                 const R nt = tab[nxt].n;
@@ -325,6 +353,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
                     ax = bx; ay = by; az = bz;
                     if (k == 0) px = -px; else if (k == 1) py = -py; else pz = -pz;
                     lab = nxt;
+                    if (SKIP) rad = nrad;
                     if (lab == end) {
                         ended = true;
                     } else {
@@ -339,6 +368,17 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
                 px += vx * sm; py += vy * sm; pz += vz * sm;
                 if (replay) grad_segment(sm);
                 else if (njac) add_path(lab, sm, (R)1);
+                if (SKIP && rad > 0) {
+                    // the hop ended inside the uniform cube: into the frame of the voxel that
+                    // holds its end; the cube around that voxel is at least rad - offset
+                    const int ox = cube_relocate<R>(px, pitch, ipitch, half, rad);
+                    const int oy = cube_relocate<R>(py, pitch, ipitch, half, rad);
+                    const int oz = cube_relocate<R>(pz, pitch, ipitch, half, rad);
+                    ax = clamp_index(ax + ox, P.nx); ay = clamp_index(ay + oy, P.ny); az = clamp_index(az + oz, P.nz);
+                    const int mx = abs(ox), my = abs(oy), mz = abs(oz);
+                    n_cross += (uint32_t)(mx + my + mz);
+                    rad -= mx > my ? (mx > mz ? mx : mz) : (my > mz ? my : mz);
+                }
                 const R wn = w - w * ra;
                 acc_abs += (double)w - (double)wn;
                 w = wn;

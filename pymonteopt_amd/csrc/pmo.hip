@@ -616,6 +616,10 @@ int diff_launch(const pmo_volume* vol, const pmo_source* src, const pmo_run* run
     pmo::DiffParams P;
     memset(&P, 0, sizeof(P));
     P.labels = vol->labels;
+    // a forward without a per-voxel field may cross the faces inside same-label cubes in one
+    // step (vol->skip); the replay and field forwards step voxel by voxel
+    const bool skip = mode == 0 && vol->skip != nullptr && diff->mua_field == nullptr;
+    P.skip = skip ? vol->skip : nullptr;
     P.optics = reinterpret_cast<const float4*>(vol->optics);
     P.mua_field = diff->mua_field;
     P.nx = (uint32_t)vol->nx; P.ny = (uint32_t)vol->ny; P.nz = (uint32_t)vol->nz;
@@ -693,8 +697,10 @@ int diff_launch(const pmo_volume* vol, const pmo_source* src, const pmo_run* run
     if (lds > d.max_lds) return fail(PMO_ERR_UNSUPPORTED, "differentiable walk needs %zu B of LDS", lds);
 
     int nb = 0;
-    hipError_t e = lines ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::diff_kernel<float, true>, bs, lds)
-                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::diff_kernel<float, false>, bs, lds);
+    hipError_t e =
+        lines  ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::diff_kernel<float, true, false>, bs, lds)
+        : skip ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::diff_kernel<float, false, true>, bs, lds)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::diff_kernel<float, false, false>, bs, lds);
     if (e != hipSuccess) return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     if (nb < 1) nb = 1;
     unsigned long long grid = run->grid_blocks > 0 ? (unsigned long long)run->grid_blocks
@@ -721,8 +727,9 @@ int diff_launch(const pmo_volume* vol, const pmo_source* src, const pmo_run* run
         e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, "zero line scratch launch");
     }
-    if (lines) hipLaunchKernelGGL((pmo::diff_kernel<float, true>), dim3((unsigned)grid), dim3(bs), lds, s, P);
-    else hipLaunchKernelGGL((pmo::diff_kernel<float, false>), dim3((unsigned)grid), dim3(bs), lds, s, P);
+    if (lines) hipLaunchKernelGGL((pmo::diff_kernel<float, true, false>), dim3((unsigned)grid), dim3(bs), lds, s, P);
+    else if (skip) hipLaunchKernelGGL((pmo::diff_kernel<float, false, true>), dim3((unsigned)grid), dim3(bs), lds, s, P);
+    else hipLaunchKernelGGL((pmo::diff_kernel<float, false, false>), dim3((unsigned)grid), dim3(bs), lds, s, P);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "differentiable walk launch");
     if (lines) {

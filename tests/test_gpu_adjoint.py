@@ -44,6 +44,34 @@ def test_path_sums_match_oracle(skin, oracle):
         assert rd.sum() == pytest.approx(o["rd_r"].sum(), rel=0.005)
 
 
+def test_skipping_forward_matches_voxel_forward(skin):
+    """forward(replay=False) crosses same-label cubes in one step: the same draws, so per
+    seed its Rd(r) and per-label path sums differ from the voxel-stepping forward's by
+    rounding only (a small fraction of the seed-to-seed spread); its record is refused by
+    the adjoint."""
+    import torch
+    from pymonteopt_amd.adjoint import DiffWalk
+    N = 200_000
+    dw = DiffWalk(skin.labels, skin.pitch, skin.optics, skin.source, n_packets=N, nr=40, dr=0.1)
+    assert dw.walker.skip                              # the skin plate's layers are uniform cubes
+    area = dw.area.cpu().numpy()
+    A, B = [], []
+    for s in range(6):
+        a = dw.forward(seed=s, jac=True, replay=False)
+        b = dw.forward(seed=s, jac=True)
+        torch.cuda.synchronize()
+        assert a["replayable"] is False and b["replayable"] is True
+        for out, acc in ((a, A), (b, B)):
+            wl = -(out["J_mua"].cpu().numpy() * area[:, None]).sum(0)          # sum W L_j / N
+            wn = (out["J_mus"].cpu().numpy() * area[:, None]).sum(0)           # sum W (n_j/mu_s - L_j) / N
+            acc.append(np.concatenate([[float((out["Rd_r"].cpu().numpy() * area).sum())], wl, wn]))
+    A, B = np.array(A), np.array(B)
+    spread = B.std(0, ddof=1)
+    assert np.all(np.abs(A - B) <= 0.5 * spread + 1e-12), (np.abs(A - B) / spread)
+    with pytest.raises(ValueError, match="replay"):
+        dw.adjoint(a, dw.area.clone())
+
+
 def _fd(dw, base, j, col, rel, seeds):
     area = dw.area
     out = []
