@@ -99,6 +99,7 @@ plate_batch_kernel(const BatchParams P)
 
     bool alive = false;
     R px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0, ivx = 0, ivy = 0, ivz = 0;
+    R imt = 0;                        // LAYER: 1 / mt of the current label
     R w = 0, st = 0, u_roul = 0, u_th = 0, u_phi = 0, fb0 = 0, fb1 = 0, fb2 = 0, half = 0;
     LabelConst<R> L;                  // optics of the current label
     PlateGeo G{nullptr, nullptr, 0, 0, 0, 0u, nullptr, 0};
@@ -155,7 +156,9 @@ plate_batch_kernel(const BatchParams P)
                         launch_xy<R>(I.src_kind, I.radius, (R)I.pitch, half, pid, 0u, seed, key1, I.nx, I.ny, ax,
                                      ay, px, py);
                     vx = (R)I.dirx; vy = (R)I.diry; vz = (R)I.dirz;
-                    ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
+                    if constexpr (!LAYER) {
+                        ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
+                    }
                     w = (R)I.w0;
                     acc_w0 += (double)I.w0;
                     // absorbed weight telescoped over the packet: each drop is exactly
@@ -184,6 +187,7 @@ plate_batch_kernel(const BatchParams P)
                         if (az < 0 || lab > G.end) lab = G.end;
                     }
                     L = label_const<R>(G.optics[lab]);
+                    if constexpr (LAYER) imt = M::rcp(L.mt);
                     inter = 0; fres = 0; nev = 0;
                     const uint4 r = philox4x32_10(make_uint4(0u, kTagWalk, pid, 0u), seed, key1);
                     st = M::neglog(u01<R>(r.x));
@@ -202,24 +206,27 @@ plate_batch_kernel(const BatchParams P)
         R db = (R)1000, smt = (R)0;
         int k = 0;
         bool want_int = false;
+        R ex = 0, ey = 0, ez = 0;             // LAYER: end point of the pending hop
         const bool walking = alive && lab != end;
         if (walking) {
-            // only components with |v_i| > 0 take part (_kernel.py:65), as in the reference
-            R d0, d1, d2;
             if constexpr (LAYER) {
-                d0 = vx != (R)0 ? (vx > (R)0 ? wxh - px : px - wxl) * ivx : (R)INFINITY;
-                d1 = vy != (R)0 ? (vy > (R)0 ? wyh - py : py - wyl) * ivy : (R)INFINITY;
-                d2 = vz != (R)0 ? (vz > (R)0 ? zb - pz : pz - zt) * ivz : (R)INFINITY;
+                // the hop of length st / mt meets no wall or layer boundary exactly when it
+                // ends inside the closed box of the current layer; its end point is then the
+                // interaction point (the fused p + v st / mt of the voxel walk)
+                const R s = st * imt;
+                ex = px + vx * s; ey = py + vy * s; ez = pz + vz * s;
+                want_int = ex >= wxl && ex <= wxh && ey >= wyl && ey <= wyh && ez >= zt && ez <= zb;
             } else {
-                d0 = vx != (R)0 ? (half - (vx > (R)0 ? px : -px)) * ivx : (R)INFINITY;
-                d1 = vy != (R)0 ? (half - (vy > (R)0 ? py : -py)) * ivy : (R)INFINITY;
-                d2 = vz != (R)0 ? (half - (vz > (R)0 ? pz : -pz)) * ivz : (R)INFINITY;
+                // only components with |v_i| > 0 take part (_kernel.py:65), as in the reference
+                const R d0 = vx != (R)0 ? (half - (vx > (R)0 ? px : -px)) * ivx : (R)INFINITY;
+                const R d1 = vy != (R)0 ? (half - (vy > (R)0 ? py : -py)) * ivy : (R)INFINITY;
+                const R d2 = vz != (R)0 ? (half - (vz > (R)0 ? pz : -pz)) * ivz : (R)INFINITY;
+                if (d0 < db) { db = d0; k = 0; }
+                if (d1 < db) { db = d1; k = 1; }
+                if (d2 < db) { db = d2; k = 2; }
+                smt = db * L.mt;
+                want_int = !(st >= smt);
             }
-            if (d0 < db) { db = d0; k = 0; }
-            if (d1 < db) { db = d1; k = 1; }
-            if (d2 < db) { db = d2; k = 2; }
-            smt = db * L.mt;
-            want_int = !(st >= smt);
         }
         const unsigned long long m_int = __ballot(walking && want_int);
         const unsigned long long m_cross = __ballot(walking && !want_int);
@@ -234,10 +241,23 @@ plate_batch_kernel(const BatchParams P)
             ++nev;
             if (!want_int) {
                 ++n_cross;
+                if constexpr (LAYER) {
+                    // distance to the wall or boundary the hop meets (_kernel.py:62-70)
+                    const R d0 = vx != (R)0 ? (vx > (R)0 ? wxh - px : px - wxl) * M::rcp(M::fabs(vx)) : (R)INFINITY;
+                    const R d1 = vy != (R)0 ? (vy > (R)0 ? wyh - py : py - wyl) * M::rcp(M::fabs(vy)) : (R)INFINITY;
+                    const R d2 = vz != (R)0 ? (vz > (R)0 ? zb - pz : pz - zt) * M::rcp(M::fabs(vz)) : (R)INFINITY;
+                    if (d0 < db) { db = d0; k = 0; }
+                    if (d1 < db) { db = d1; k = 1; }
+                    if (d2 < db) { db = d2; k = 2; }
+                    smt = db * L.mt;
+                }
                 px += vx * db; py += vy * db; pz += vz * db;
                 const R vk = k == 0 ? vx : (k == 1 ? vy : vz);
                 const int s = vk > (R)0 ? 1 : -1;
                 st -= smt;
+                // where the end-point test and st >= db mt round to different answers the
+                // remainder is a few ulps below zero: the next hop starts on the face
+                if constexpr (LAYER) st = M::fmax(st, (R)0);
                 uint32_t nxt = end;
                 int32_t bx = ax, by = ay, bz = az;
                 R nzt = zt, nzb = zb;
@@ -280,7 +300,9 @@ plate_batch_kernel(const BatchParams P)
                         const R rf = fresnel_reflectance<R>(L.n, Ln.n, M::fabs(vk), normal_limit, ct);
                         if (rf < u) {
                             refract<R>(k, vk, L.n / Ln.n, ct, vx, vy, vz);
-                            ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
+                            if constexpr (!LAYER) {
+                                ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
+                            }
                         } else {
                             transmit = false;
                         }
@@ -297,14 +319,19 @@ plate_batch_kernel(const BatchParams P)
                         ended = true; xk = k; xs = s;
                     } else if (nxt != lab) {
                         L = Ln;
+                        if constexpr (LAYER) imt = M::rcp(L.mt);
                     }
                     lab = nxt;
                 } else {
                     if (k == 0) vx = -vx; else if (k == 1) vy = -vy; else vz = -vz;
                 }
             } else {
-                const R sm = M::div(st, L.mt);
-                px += vx * sm; py += vy * sm; pz += vz * sm;
+                if constexpr (LAYER) {
+                    px = ex; py = ey; pz = ez;
+                } else {
+                    const R sm = M::div(st, L.mt);
+                    px += vx * sm; py += vy * sm; pz += vz * sm;
+                }
                 w = w - w * L.ra;
                 if (w <= (R)P.wth) {
                     atomicAdd(&bev[kBE_ROULETTE], 1ull);
@@ -320,7 +347,9 @@ plate_batch_kernel(const BatchParams P)
                 }
                 if (!killed) {
                     spin<R>(hg_cos<R>(L, u_th), u_phi, vx, vy, vz);
-                    ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
+                    if constexpr (!LAYER) {
+                        ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
+                    }
                     ++inter;
                     const uint4 r = philox4x32_10(make_uint4(inter, kTagWalk, pid, 0u), seed, key1);
                     st = M::neglog(u01<R>(r.x));
