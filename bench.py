@@ -314,10 +314,18 @@ def bone_leg(peaks, n_packets=100_000_000, wide=True):
     this the crossing- and Fresnel-heavy case. wide: the 256^3 phantom's core tiled 5 x 5
     (1072 x 1072 x 256 voxels, 21.4 mm wide) with the rings centred at 1..10 mm; else the
     256^3 phantom itself with rings at 0.25..2.5 mm."""
-    import torch
-    from pymonteopt_amd import _abi, presets, roofline
-    from pymonteopt_amd.runtime import TallySpec, Walker
+    from pymonteopt_amd import presets
     wl = presets.bone_wide_workload(seed=0) if wide else presets.bone_workload(seed=0)
+    return workload_leg(wl, peaks, n_packets)
+
+
+def workload_leg(wl, peaks, n_packets=100_000_000):
+    """One launch of n_packets through a workload with all its tallies and the absorbed map,
+    at the launch shape Walker.tune picks, after a 1e6-packet warm-up; photons/s from HIP
+    events around the launch, with its VALU and atomic roofline fractions."""
+    import torch
+    from pymonteopt_amd import _abi, roofline
+    from pymonteopt_amd.runtime import TallySpec, Walker
     wk = Walker(wl.labels, wl.pitch, wl.optics)
     spec = TallySpec(**wl.tallies)
     kw, _ = wk.tune(wl.source, spec, n_packets=5_000_000)
@@ -335,15 +343,49 @@ def bone_leg(peaks, n_packets=100_000_000, wide=True):
     ev = dict(zip(_abi.EVENT_NAMES, [int(x) for x in tal.events.cpu().tolist()]))
     ops = roofline.ops_per_packet(ev)
     glob_pp = ev["global_deposits"] / n_packets
-    rings = tal.hist["ring_w"].cpu().numpy() / n_packets
-    return {"workload": wl.name + f", {n_packets:.0e} packets", "seconds": t, "photons_per_s": n_packets / t,
-            "launch": kw or "library default", "rings_mm": [list(r) for r in wl.tallies["rings"]],
-            "events_per_packet": {k: ev[k] / n_packets for k in ("interactions", "crossings", "fresnel", "roulette")},
-            "Rd": s["Rd"] / n_packets, "Tt": s["Tt"] / n_packets, "A": s["A"] / n_packets,
-            "ring_weight_per_packet": [float(x) for x in rings],
-            "valu_frac_published": n_packets * ops / t / roofline.PEAK_VALU_PUBLISHED_LANE_OPS,
-            "atomic_frac_measured": glob_pp * n_packets / t / peaks["fp64_atomics_per_s"],
-            "global_deposits_per_packet": glob_pp, "ops_per_packet": ops}
+    out = {"workload": wl.name + f", {n_packets:.0e} packets", "seconds": t, "photons_per_s": n_packets / t,
+           "launch": kw or "library default",
+           "events_per_packet": {k: ev[k] / n_packets for k in ("interactions", "crossings", "fresnel", "roulette")},
+           "Rd": s["Rd"] / n_packets, "Tt": s["Tt"] / n_packets, "A": s["A"] / n_packets,
+           "valu_frac_published": n_packets * ops / t / roofline.PEAK_VALU_PUBLISHED_LANE_OPS,
+           "atomic_frac_measured": glob_pp * n_packets / t / peaks["fp64_atomics_per_s"],
+           "global_deposits_per_packet": glob_pp, "ops_per_packet": ops}
+    if wl.tallies.get("rings"):
+        out["rings_mm"] = [list(r) for r in wl.tallies["rings"]]
+        out["ring_weight_per_packet"] = [float(x) for x in tal.hist["ring_w"].cpu().numpy() / n_packets]
+    return out
+
+
+def slab_leg(n_batches=10, n_packets=100_000):
+    """The CPU-runnable slab (101 x 101 x 50 @0.1 mm, pencil beam, absorbed map and Rd(r) in
+    50 bins of 0.1 mm) as its ten 1e5-packet seed batches (seeds 0-9), each a launch of its
+    own: the batches' total GPU-event time, and Rd, Tt, A with their standard errors over the
+    batches."""
+    import numpy as np
+    import torch
+    from pymonteopt_amd import _abi, presets
+    from pymonteopt_amd.runtime import TallySpec, Walker
+    wl = presets.slab_plate()
+    wk = Walker(wl.labels, wl.pitch, wl.optics)
+    tal = wk.tallies(TallySpec(**wl.tallies))
+    wk.walk(wl.source, n_packets, 99, tal)                  # code-object load
+    torch.cuda.synchronize()
+    t, res = 0.0, []
+    for seed in range(n_batches):
+        tal.zero()
+        e0, e1 = _events()
+        e0.record()
+        wk.walk(wl.source, n_packets, seed, tal)
+        e1.record()
+        e1.synchronize()
+        t += e0.elapsed_time(e1) * 1e-3
+        s = dict(zip(_abi.SCALAR_NAMES, tal.scalars.cpu().tolist()))
+        res.append([s["Rd"] / n_packets, s["Tt"] / n_packets, s["A"] / n_packets])
+    r = np.array(res)
+    m, se = r.mean(0), r.std(0, ddof=1) / math.sqrt(n_batches)
+    return {"workload": wl.name + f", seeds 0-{n_batches - 1} x {n_packets:.0e} packets", "seconds": t,
+            "photons_per_s": n_batches * n_packets / t, "ms_per_batch": 1e3 * t / n_batches,
+            "Rd": [float(m[0]), float(se[0])], "Tt": [float(m[1]), float(se[1])], "A": [float(m[2]), float(se[2])]}
 
 
 # The reference's own published benchmark: a homogeneous 5 mm slab, 5 x 5 mm wide (mu_s 10,
@@ -713,12 +755,15 @@ def main():
             out["obd"], obd_sample = obd_leg()
             out["turing"] = turing_leg()
             out["virtual_obd"] = virtual_obd_leg()
+            out["skin_flat"] = workload_leg(presets.skin_plate("flat"), peaks)
+            out["slab"] = slab_leg()
             out["bone"] = bone_leg(peaks, wide=True)
             out["bone_256"] = bone_leg(peaks, wide=False)
             out["published_slab"] = published_slab_leg()
         if not args.no_cpu:
             out["cpu_baseline"] = cpu_baseline(wl)
             if not args.no_extra:
+                out["slab"]["gpu_vs_cpu"] = out["slab"]["photons_per_s"] / out["cpu_baseline"]["slab_101x101x50"]["value"]
                 out["obd"]["cpu_baseline"] = cpu_baseline_obd(*obd_sample)
                 out["obd"]["gpu_vs_cpu"] = out["obd"]["traces_per_s"] / out["obd"]["cpu_baseline"]["value"]
                 out["turing"]["cpu_baseline"] = cpu_baseline_turing()
