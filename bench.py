@@ -166,7 +166,10 @@ def batched_leg(n_inst=512):
     tp, rd_p = sequential(None)
     tpv, rd_pv = sequential(False)
     strip = lambda r: {k: v for k, v in r.items() if k != "table"}
+    ev = rb["events"]
+    per_packet = {k: ev[k] / max(ev["packets"], 1) for k in ("interactions", "crossings", "fresnel")}
     return {"instances": n_inst, "packets": N, "stepping": "layer", "batched_s": tb, "batched_photons_per_s": N / tb,
+            "events_per_packet": per_packet,
             "sequential_plain_s": tp, "sequential_photons_per_s": N / tp, "speedup": tp / tb,
             "max_abs_Rd_diff_same_draws": float(np.max(np.abs(rd_p - rb["Rd"]))), "tuned": strip(tuned),
             "sequential_plain_voxel_stepping_s": tpv, "speedup_vs_voxel_stepping_plain": tpv / tb,
@@ -547,7 +550,8 @@ def cpu_baseline_turing(n=128, steps=4):
 
 def measure_peaks():
     """Re-measures the two peaks with libpmo's probes: FP64 global atomic adds per second
-    (pseudo-random addresses in 32 MiB) and FP32 FMA lane-ops per second."""
+    (pseudo-random addresses in 32 MiB) and FP32 FMA lane-ops per second; and the rate of
+    Philox4x32-10 blocks (the walk's draws) with every lane active."""
     import ctypes as C
     import torch
     from pymonteopt_amd import _abi
@@ -573,6 +577,14 @@ def measure_peaks():
         e1.record()
         e1.synchronize()
         out["valu_lane_ops_per_s"] = vb * 256.0 * vi * 32 / (e0.elapsed_time(e1) * 1e-3)
+    w = torch.empty(vb * 256, dtype=torch.int32, device="cuda:0")
+    for _ in range(2):
+        e0, e1 = _events()
+        e0.record()
+        _abi.check(L.pmo_probe_philox(C.c_void_p(w.data_ptr()), vb, 1024, s))
+        e1.record()
+        e1.synchronize()
+        out["philox_blocks_per_s"] = vb * 256.0 * 1024 / (e0.elapsed_time(e1) * 1e-3)
     return out
 
 
@@ -751,6 +763,12 @@ def main():
     with contextlib.redirect_stdout(sys.stderr):
         if not args.no_extra:
             out["batched"] = batched_leg()
+            # the sweep draws one Philox block per interaction: its rate of blocks against the
+            # probe's rate with every lane active is the share of the device the draws take
+            b = out["batched"]
+            blocks_s = b["packets"] * b["events_per_packet"]["interactions"] / b["batched_s"]
+            b["draws"] = {"philox_blocks_per_s": blocks_s, "probe_blocks_per_s": peaks["philox_blocks_per_s"],
+                          "share_of_probe_rate": blocks_s / peaks["philox_blocks_per_s"]}
             out["adjoint"] = adjoint_leg()
             out["obd"], obd_sample = obd_leg()
             out["turing"] = turing_leg()

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define PMO_ABI_VERSION 9
+#define PMO_ABI_VERSION 10
 
 /* ---- error codes ---------------------------------------------------------------- */
 #define PMO_OK 0
@@ -342,7 +342,9 @@ int pmo_walk_adjoint(const pmo_volume* vol, const pmo_source* src, const pmo_run
  * pseudo-random 512 B block of buf[n], n >= 64; with PMO_PROBE_F64_BLOCKS_32B / _64B / _128B
  * the lanes in groups of 4 / 8 / 16 each add to the consecutive doubles of their own
  * pseudo-random 32 / 64 / 128 B-aligned block (n >= 16). pmo_probe_valu runs blocks*256 lanes that
- * each issue iters x 32 independent FP32 FMAs and write one float to out[blocks*256]. */
+ * each issue iters x 32 independent FP32 FMAs and write one float to out[blocks*256].
+ * pmo_probe_philox runs blocks*256 lanes that each compute iters Philox4x32-10 blocks (the
+ * walk's draws, one block per interaction) and write one word to out[blocks*256]. */
 enum {
     PMO_PROBE_F64 = 0,
     PMO_PROBE_F32 = 1,
@@ -357,6 +359,7 @@ enum {
 };
 int pmo_probe_atomics(void* buf, int64_t n, int32_t blocks, int32_t iters, int32_t kind, void* stream);
 int pmo_probe_valu(float* out, int32_t blocks, int32_t iters, void* stream);
+int pmo_probe_philox(uint32_t* out, int32_t blocks, int32_t iters, void* stream);
 
 /* Host helper: the round-robin batch schedule of a sweep. Writes up to `cap` batches
  * (instance, first local packet id, count, 0) into `sched` (host, 4*cap uint32), cycling over

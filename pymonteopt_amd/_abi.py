@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpmo.so")
 
-ABI_VERSION = 9          # PMO_ABI_VERSION of include/pmo.h
+ABI_VERSION = 10         # PMO_ABI_VERSION of include/pmo.h
 
 PMO_OK = 0
 PMO_ERR_INVALID = -1
@@ -146,6 +146,7 @@ SIGNATURES = {
                        C.c_void_p, C.c_void_p], C.c_int),
     "pmo_probe_atomics": ([C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_void_p], C.c_int),
     "pmo_probe_valu": ([C.c_void_p, C.c_int32, C.c_int32, C.c_void_p], C.c_int),
+    "pmo_probe_philox": ([C.c_void_p, C.c_int32, C.c_int32, C.c_void_p], C.c_int),
     "pmo_plate_schedule": ([C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int64], C.c_int64),
     "pmo_walk_plates": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.POINTER(PmoRun),
                          C.POINTER(PmoBatchTallies), C.c_void_p, C.c_void_p], C.c_int),

@@ -559,6 +559,16 @@ int pmo_probe_valu(float* out, int32_t blocks, int32_t iters, void* stream)
     return e == hipSuccess ? PMO_OK : hip_fail(e, "probe launch");
 }
 
+int pmo_probe_philox(uint32_t* out, int32_t blocks, int32_t iters, void* stream)
+{
+    g_err[0] = '\0';
+    if (!out || blocks < 1 || iters < 1) return fail(PMO_ERR_INVALID, "bad probe size");
+    hipLaunchKernelGGL(pmo::probe_philox_kernel, dim3(blocks), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       out, iters, 1u);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PMO_OK : hip_fail(e, "probe launch");
+}
+
 int pmo_philox(const uint32_t* ctr, uint32_t key0, uint32_t key1, uint32_t* out, int64_t n, void* stream)
 {
     g_err[0] = '\0';

@@ -987,6 +987,19 @@ __global__ void probe_valu_kernel(float* out, int iters)
     out[t] = s;
 }
 
+// Philox4x32-10 blocks back to back, as the walk draws them (counter = block, tag, lane id),
+// folded into one word per lane so that none is optimised away
+__global__ void probe_philox_kernel(uint32_t* out, int iters, uint32_t seed)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t acc = 0u;
+    for (int i = 0; i < iters; ++i) {
+        const uint4 r = philox4x32_10(make_uint4((uint32_t)i, kTagWalk, t, 0u), seed, 7u);
+        acc ^= r.x + r.y + r.z + r.w;
+    }
+    out[t] = acc;
+}
+
 __global__ void philox_kernel(const uint4* ctr, uint32_t k0, uint32_t k1, uint4* out, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

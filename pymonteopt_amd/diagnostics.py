@@ -86,6 +86,11 @@ def peaks():
     o = torch.empty(2048 * 256, dtype=torch.float32, device="cuda:0")
     t = timed(lambda: _abi.check(L.pmo_probe_valu(C.c_void_p(o.data_ptr()), 2048, 8192, s)))
     out["valu_fp32_fma_lane_ops_per_s"] = 2048 * 256.0 * 8192 * 32 / t
+    # the walk's draws: Philox4x32-10 blocks per second with every lane active
+    w = torch.empty(2048 * 256, dtype=torch.int32, device="cuda:0")
+    t = timed(lambda: _abi.check(L.pmo_probe_philox(C.c_void_p(w.data_ptr()), 2048, 1024, s)))
+    out["philox4x32_10_blocks_per_s"] = 2048 * 256.0 * 1024 / t
+    out["philox_fma_slots_per_block"] = out["valu_fp32_fma_lane_ops_per_s"] / out["philox4x32_10_blocks_per_s"]
     return out
 
 

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), n
         assert n in _abi.SIGNATURES, n
-    assert L.pmo_abi_version() == _abi.ABI_VERSION == 9
+    assert L.pmo_abi_version() == _abi.ABI_VERSION == 10
     assert L.pmo_workspace_bytes() >= 8
 
 
