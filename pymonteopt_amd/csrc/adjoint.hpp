@@ -111,6 +111,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
     int32_t rad = 0;                 // SKIP: uniform-cube radius around the voxel, a lower bound
     uint32_t lab = 0, inter = 0, fres = 0, nev = 0;
     unsigned long long pid = 0;
+    PhiloxPacket pq{0u, 0u, 0u, 0u};  // the packet's part of its interaction draws
     double acc_abs = 0.0;
     // replay: pending gradient deposit and this packet's -c_b W. The slot is -(box cell)-1 in
     // the LDS box; outside it the voxel index, or with line buffers 8 x cube + element
@@ -245,7 +246,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
                         if (lab != end) voxel_optics(lab, ax, ay, az);
                         inter = 0; fres = 0; nev = 0;
                         L0 = L1 = L2 = L3 = N0 = N1 = N2 = N3 = (R)0;
-                        const uint4 r = philox4x32_10(make_uint4(0u, kTagWalk, plo, phi), P.seed, P.stream);
+                        pq = philox_packet(kTagWalk, plo, P.seed);
+                        const uint4 r = philox_packet_block(pq, 0u, phi, P.seed, P.stream);
                         st = M::neglog(u01<R>(r.x));
                         u_roul = u01<R>(r.y); u_th = u01<R>(r.z); u_phi = u01<R>(r.w);
                         alive = true;
@@ -396,8 +398,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
                     spin<R>(hg_cos<R>(tab[lab], u_th), u_phi, vx, vy, vz);
                     ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
                     ++inter;
-                    const uint4 r = philox4x32_10(make_uint4(inter, kTagWalk, (uint32_t)pid,
-                                                             (uint32_t)(pid >> 32)), P.seed, P.stream);
+                    const uint4 r = philox_packet_block(pq, inter, (uint32_t)(pid >> 32), P.seed, P.stream);
                     st = M::neglog(u01<R>(r.x));
                     u_roul = u01<R>(r.y); u_th = u01<R>(r.z); u_phi = u01<R>(r.w);
                 } else {

@@ -106,6 +106,7 @@ plate_batch_kernel(const BatchParams P)
     R zt = 0, zb = 0, wxl = 0, wxh = 0, wyl = 0, wyh = 0;   // LAYER: current layer, lateral walls
     int32_t ax = 0, ay = 0, az = 0;
     uint32_t lab = 0, inter = 0, fres = 0, nev = 0, pid = 0, key1 = 0;
+    PhiloxPacket pq{0u, 0u, 0u, 0u};  // the packet's part of its interaction draws
     int32_t inst = -1;
     double acc_abs = 0.0, acc_w0 = 0.0;
     uint32_t n_cross = 0;
@@ -189,7 +190,8 @@ plate_batch_kernel(const BatchParams P)
                     L = label_const<R>(G.optics[lab]);
                     if constexpr (LAYER) imt = M::rcp(L.mt);
                     inter = 0; fres = 0; nev = 0;
-                    const uint4 r = philox4x32_10(make_uint4(0u, kTagWalk, pid, 0u), seed, key1);
+                    pq = philox_packet(kTagWalk, pid, seed);
+                    const uint4 r = philox_packet_block(pq, 0u, 0u, seed, key1);
                     st = M::neglog(u01<R>(r.x));
                     u_roul = u01<R>(r.y); u_th = u01<R>(r.z); u_phi = u01<R>(r.w);
                     alive = true;
@@ -351,7 +353,7 @@ plate_batch_kernel(const BatchParams P)
                         ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
                     }
                     ++inter;
-                    const uint4 r = philox4x32_10(make_uint4(inter, kTagWalk, pid, 0u), seed, key1);
+                    const uint4 r = philox_packet_block(pq, inter, 0u, seed, key1);
                     st = M::neglog(u01<R>(r.x));
                     u_roul = u01<R>(r.y); u_th = u01<R>(r.z); u_phi = u01<R>(r.w);
                 } else {

@@ -36,6 +36,54 @@ __host__ __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, ui
     return c;
 }
 
+// A packet's blocks of one tag, counter (block, tag, plo, phi): round 1's x and y words and
+// round 2's product M0 * x depend only on the tag, plo and the key, not on the block index.
+// philox_packet computes them once per packet; philox_packet_block then gives block `blk`
+// for 16 of the 18 64-bit multiply-adds (the quarter-rate instructions of a block), equal
+// to philox4x32_10(make_uint4(blk, tag, plo, phi), k0, k1) bit for bit.
+struct PhiloxPacket {
+    uint32_t x1, y1;   // round 1: hi(M1 plo) ^ tag ^ k0, lo(M1 plo)
+    uint32_t h2, w2;   // round 2: hi(M0 x1), lo(M0 x1)
+};
+
+__host__ __device__ __forceinline__ PhiloxPacket philox_packet(uint32_t tag, uint32_t plo, uint32_t k0)
+{
+    const uint64_t m1 = (uint64_t)kPhiloxM1 * plo;
+    PhiloxPacket q;
+    q.x1 = (uint32_t)(m1 >> 32) ^ tag ^ k0;
+    q.y1 = (uint32_t)m1;
+    const uint64_t m0 = (uint64_t)kPhiloxM0 * q.x1;
+    q.h2 = (uint32_t)(m0 >> 32);
+    q.w2 = (uint32_t)m0;
+    return q;
+}
+
+__host__ __device__ __forceinline__ uint4 philox_packet_block(const PhiloxPacket& q, uint32_t blk, uint32_t phi,
+                                                              uint32_t k0, uint32_t k1)
+{
+    // round 1: only the product of the block index is new
+    const uint64_t a = (uint64_t)kPhiloxM0 * blk;
+    const uint32_t z1 = (uint32_t)(a >> 32) ^ phi ^ k1;
+    const uint32_t w1 = (uint32_t)a;
+    k0 += kPhiloxW0;
+    k1 += kPhiloxW1;
+    // round 2: M0 x1 comes from the packet
+    const uint64_t b = (uint64_t)kPhiloxM1 * z1;
+    uint4 c = make_uint4((uint32_t)(b >> 32) ^ q.y1 ^ k0, (uint32_t)b, q.h2 ^ w1 ^ k1, q.w2);
+    k0 += kPhiloxW0;
+    k1 += kPhiloxW1;
+#pragma unroll
+    for (int r = 2; r < 10; ++r) {
+        const uint64_t m0 = (uint64_t)kPhiloxM0 * c.x;
+        const uint64_t m1 = (uint64_t)kPhiloxM1 * c.z;
+        c = make_uint4((uint32_t)(m1 >> 32) ^ c.y ^ k0, (uint32_t)m1,
+                       (uint32_t)(m0 >> 32) ^ c.w ^ k1, (uint32_t)m0);
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
+    }
+    return c;
+}
+
 // uniform on (0, 1]: top 24 bits plus one, times 2^-24; exact in float and double, never 0
 // so -log(u) is finite (the reference's curand_uniform range, _cukernel.py:75).
 // Written as k 2^-24 + 2^-24 with k = x >> 8 < 2^24: both steps are exact, so the value is

@@ -74,7 +74,10 @@ def test_spin_keeps_unit_length_and_turns_by_theta(oracle, v, c, phi):
     v = np.array(v) / np.linalg.norm(v)
     w = oracle.spin(v, c, phi)
     assert np.linalg.norm(w) == pytest.approx(1.0, abs=1e-12)
-    assert float(np.dot(v, w)) == pytest.approx(c, abs=1e-6)
+    # above |v_z| = 0.99999 the reference turns v as if it were (0, 0, sign v_z)
+    # (_kernel.py:149-153), off by at most the tilt it ignores
+    tilt = 0.0 if abs(v[2]) <= 0.99999 else 2.0 * math.sqrt(max(0.0, 1.0 - v[2] ** 2))
+    assert float(np.dot(v, w)) == pytest.approx(c, abs=1e-6 + tilt)
 
 
 @SET
