@@ -316,21 +316,30 @@ def bone_leg(peaks, n_packets=100_000_000, wide=True):
     top-face detectors, after a 1e6-packet warm-up. Fresnel at every bone/marrow face makes
     this the crossing- and Fresnel-heavy case. wide: the 256^3 phantom's core tiled 5 x 5
     (1072 x 1072 x 256 voxels, 21.4 mm wide) with the rings centred at 1..10 mm; else the
-    256^3 phantom itself with rings at 0.25..2.5 mm."""
+    256^3 phantom itself with rings at 0.25..2.5 mm. The bone workload's outputs are the
+    detectors, Rd(r) and the scalars; the walk's own per-voxel absorbed map (which the
+    reference kernel does not tally) is timed with them, and the same launch without it is
+    reported beside (`detectors_only`)."""
     from pymonteopt_amd import presets
+    from pymonteopt_amd.runtime import Walker
     wl = presets.bone_wide_workload(seed=0) if wide else presets.bone_workload(seed=0)
-    return workload_leg(wl, peaks, n_packets)
+    wk = Walker(wl.labels, wl.pitch, wl.optics)
+    out = workload_leg(wl, peaks, n_packets, walker=wk)
+    out["detectors_only"] = workload_leg(wl, peaks, n_packets, walker=wk, absorbed=False)
+    return out
 
 
-def workload_leg(wl, peaks, n_packets=100_000_000):
-    """One launch of n_packets through a workload with all its tallies and the absorbed map,
-    at the launch shape Walker.tune picks, after a 1e6-packet warm-up; photons/s from HIP
-    events around the launch, with its VALU and atomic roofline fractions."""
+def workload_leg(wl, peaks, n_packets=100_000_000, walker=None, absorbed=True):
+    """One launch of n_packets through a workload with all its tallies (and the absorbed
+    map unless absorbed=False), at the launch shape Walker.tune picks, after a 1e6-packet
+    warm-up; photons/s from HIP events around the launch, with its VALU and atomic roofline
+    fractions."""
     import torch
     from pymonteopt_amd import _abi, roofline
     from pymonteopt_amd.runtime import TallySpec, Walker
-    wk = Walker(wl.labels, wl.pitch, wl.optics)
+    wk = walker if walker is not None else Walker(wl.labels, wl.pitch, wl.optics)
     spec = TallySpec(**wl.tallies)
+    spec.absorbed = bool(absorbed)
     kw, _ = wk.tune(wl.source, spec, n_packets=5_000_000)
     tal = wk.tallies(spec)
     wk.walk(wl.source, 1_000_000, 99, tal, **kw)
