@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define PMO_ABI_VERSION 10
+#define PMO_ABI_VERSION 11
 
 /* ---- error codes ---------------------------------------------------------------- */
 #define PMO_OK 0
@@ -111,6 +111,11 @@ typedef struct pmo_volume {
                               one step (the same draws; merged arithmetic, rounded once); NULL
                               steps voxel by voxel as the reference does. pmo_walk_forward_diff
                               uses it without mua_field; other entry points ignore it.        */
+    const uint16_t* label_radius; /* device [nx][ny][nz], labels[v] | skip[v] << 8 of each voxel
+                              (pmo_pack_label_radius), or NULL. With skip, pmo_walk reads a
+                              voxel's label and radius from it in one 2-byte load (one memory
+                              access per crossing instead of two); it must hold exactly labels
+                              and skip. Other entry points ignore it. No reference counterpart. */
 } pmo_volume;
 
 typedef struct pmo_source {
@@ -292,6 +297,11 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run,
  * must be serialised by the caller. Arguments, results and errors are pmo_walk's. */
 int pmo_walk_noalloc(const pmo_volume* vol, const pmo_source* src, const pmo_run* run,
                      const pmo_tallies* tal, void* stream);
+
+/* out[v] = labels[v] | radius[v] << 8 for v < n (device buffers; out holds n uint16): the
+ * packed form of a label volume and its pmo_skip_radius that pmo_volume.label_radius takes.
+ * No reference counterpart. */
+int pmo_pack_label_radius(const uint8_t* labels, const uint8_t* radius, uint16_t* out, int64_t n, void* stream);
 
 /* Largest radius pmo_skip_radius accepts */
 #define PMO_SKIP_MAX_CAP 127

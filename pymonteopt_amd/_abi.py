@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpmo.so")
 
-ABI_VERSION = 10         # PMO_ABI_VERSION of include/pmo.h
+ABI_VERSION = 11         # PMO_ABI_VERSION of include/pmo.h
 
 PMO_OK = 0
 PMO_ERR_INVALID = -1
@@ -44,7 +44,8 @@ PMO_EV_COUNT = 8
 
 class PmoVolume(C.Structure):
     _fields_ = [("labels", C.c_void_p), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
-                ("n_labels", C.c_int32), ("pitch", C.c_float), ("optics", C.c_void_p), ("skip", C.c_void_p)]
+                ("n_labels", C.c_int32), ("pitch", C.c_float), ("optics", C.c_void_p), ("skip", C.c_void_p),
+                ("label_radius", C.c_void_p)]
 
 
 class PmoSource(C.Structure):
@@ -127,6 +128,7 @@ SIGNATURES = {
     "pmo_walk_noalloc": ([C.POINTER(PmoVolume), C.POINTER(PmoSource), C.POINTER(PmoRun), C.POINTER(PmoTallies),
                           C.c_void_p], C.c_int),
     "pmo_philox": ([C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
+    "pmo_pack_label_radius": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
     "pmo_skip_radius": ([C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                          C.c_void_p], C.c_int),
     "pmo_walk_forward_diff": ([C.POINTER(PmoVolume), C.POINTER(PmoSource), C.POINTER(PmoRun), C.c_void_p,

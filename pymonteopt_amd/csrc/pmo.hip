@@ -248,6 +248,7 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     memset(&P, 0, sizeof(P));
     P.labels = vol->labels;
     P.skip = vol->skip;
+    P.label_radius = vol->skip ? vol->label_radius : nullptr;
     P.optics = reinterpret_cast<const float4*>(vol->optics);
     P.nx = (uint32_t)vol->nx; P.ny = (uint32_t)vol->ny; P.nz = (uint32_t)vol->nz;
     P.nvox = (uint32_t)nvox;
@@ -459,6 +460,19 @@ int pmo_walk_noalloc(const pmo_volume* vol, const pmo_source* src, const pmo_run
     int rc = scratch_address(&ws);
     if (rc != PMO_OK) return rc;
     return pmo_walk(vol, src, run, tal, ws, stream);
+}
+
+int pmo_pack_label_radius(const uint8_t* labels, const uint8_t* radius, uint16_t* out, int64_t n, void* stream)
+{
+    g_err[0] = '\0';
+    if (n < 0) return fail(PMO_ERR_INVALID, "n < 0");
+    if (n == 0) return PMO_OK;
+    if (!labels || !radius || !out) return fail(PMO_ERR_INVALID, "labels, radius and out must be non-null");
+    const unsigned long long blocks = ((unsigned long long)n + 255ull) / 256ull;
+    hipLaunchKernelGGL(pmo::pack_label_radius_kernel, dim3((unsigned)(blocks < 65536ull ? blocks : 65536ull)), dim3(256),
+                       0, reinterpret_cast<hipStream_t>(stream), labels, radius, out, (unsigned long long)n);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PMO_OK : hip_fail(e, "pack launch");
 }
 
 int pmo_skip_radius(const uint8_t* labels, int32_t nx, int32_t ny, int32_t nz, int32_t cap, uint8_t* radius,

@@ -83,6 +83,7 @@ template <typename R> struct LabelConst {
 struct WalkParams {
     const uint8_t* labels;
     const uint8_t* skip;       // uniform-cube radius per voxel (skip_*_kernel), or null
+    const uint16_t* label_radius;  // label | radius << 8 per voxel (with skip), or null
     const float4* optics;
     uint32_t nx, ny, nz, nvox;
     uint32_t sx, sy;           // strides of x and y in the C-order volume
@@ -501,8 +502,14 @@ __device__ __forceinline__ void walk_body(const WalkParams& P)
                     rad = 0;
                     if ((uint32_t)ax < P.nx && (uint32_t)ay < P.ny && (uint32_t)az < P.nz) {
                         const uint32_t li = ((uint32_t)ax * P.ny + (uint32_t)ay) * P.nz + (uint32_t)az;
-                        l0 = P.labels[li];
-                        if (SKIP) rad = P.skip[li];
+                        if (SKIP && P.label_radius) {
+                            const uint32_t lr = P.label_radius[li];
+                            l0 = lr & 255u;
+                            rad = (int32_t)(lr >> 8);
+                        } else {
+                            l0 = P.labels[li];
+                            if (SKIP) rad = P.skip[li];
+                        }
                     }
                     lab = l0 < end ? l0 : end;
                     ni = tab[lab].n;
@@ -592,8 +599,15 @@ __device__ __forceinline__ void walk_body(const WalkParams& P)
                 int32_t nrad = 0;
                 if ((uint32_t)bx < P.nx && (uint32_t)by < P.ny && (uint32_t)bz < P.nz) {
                     const uint32_t nvi = (__umul24((uint32_t)bx, P.ny) + (uint32_t)by) * P.nz + (uint32_t)bz;
-                    nxt = P.labels[nvi];
-                    if (SKIP) nrad = P.skip[nvi];
+                    if (SKIP && P.label_radius) {
+                        // label and radius in one 2-byte load: one memory access per crossing
+                        const uint32_t lr = P.label_radius[nvi];
+                        nxt = lr & 255u;
+                        nrad = (int32_t)(lr >> 8);
+                    } else {
+                        nxt = P.labels[nvi];
+                        if (SKIP) nrad = P.skip[nvi];
+                    }
                     nxt = nxt < end ? nxt : end;
                 }
                 const R nt = tab[nxt].n;
@@ -905,6 +919,14 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n)
 }
 
 // clears n doubles (a fill kernel, so the walk can be graph-captured)
+__global__ void __launch_bounds__(256) pack_label_radius_kernel(const uint8_t* labels, const uint8_t* radius,
+                                                                 uint16_t* out, unsigned long long n)
+{
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        out[i] = (uint16_t)((uint32_t)labels[i] | ((uint32_t)radius[i] << 8));
+}
+
 __global__ void __launch_bounds__(256) zero_f64_kernel(double* p, unsigned long long n)
 {
     const unsigned long long step = (unsigned long long)gridDim.x * blockDim.x;

@@ -179,12 +179,15 @@ class Walker:
         self.skip = self.nvox > SKIP_MIN_VOXELS if skip is None else bool(skip)
         self.skip_cap = int(skip_cap)
         self._skip_radius = None
+        self._label_radius = None
         if self.skip:
             r = self.skip_radius()      # part of the geometry upload, outside any timed walk
             # an integer tensor's sum accumulates in int64 (an explicit dtype would cast a copy)
             if skip is None and int(r.sum()) < SKIP_MIN_MEAN_RADIUS * self.nvox:
                 self.skip = False       # too fragmented to gain
                 self._skip_radius = None
+            else:
+                self.label_radius()
 
     def skip_radius(self) -> torch.Tensor:
         """The per-voxel uniform-cube radius (uint8, the label volume's layout), computed on
@@ -200,6 +203,21 @@ class Walker:
                 st.synchronize()
             self._skip_radius = out
         return self._skip_radius
+
+    def label_radius(self) -> torch.Tensor:
+        """Labels and uniform-cube radius packed per voxel (label | radius << 8, uint16 as
+        int16 storage), computed on the device once: the skipping walk reads both with one
+        2-byte load per crossing."""
+        if self._label_radius is None:
+            rad = self.skip_radius()
+            out = torch.empty(self.nvox, dtype=torch.int16, device=self.device)
+            with torch.cuda.device(self.device):
+                st = torch.cuda.current_stream(self.device)
+                _abi.check(self.lib.pmo_pack_label_radius(_ptr(self.labels), _ptr(rad), _ptr(out), self.nvox,
+                                                          C.c_void_p(st.cuda_stream)))
+                st.synchronize()
+            self._label_radius = out
+        return self._label_radius
 
     def _validate_on_device(self):
         v = self.labels.view(self.shape)
@@ -226,7 +244,8 @@ class Walker:
         return TallySet(spec, self.shape, n_packets, self.device)
 
     def volume_struct(self, skip=False) -> _abi.PmoVolume:
-        """The volume for the C ABI; skip=True adds the uniform-cube radius (pmo_walk only)."""
+        """The volume for the C ABI; skip=True adds the uniform-cube radius and its packed
+        form with the labels (pmo_walk reads that; the differentiable walk the radius)."""
         v = _abi.PmoVolume()
         v.labels = _ptr(self.labels)
         v.nx, v.ny, v.nz = self.shape
@@ -234,6 +253,7 @@ class Walker:
         v.pitch = self.pitch
         v.optics = _ptr(self.optics)
         v.skip = _ptr(self.skip_radius()) if skip else None
+        v.label_radius = _ptr(self.label_radius()) if skip else None
         return v
 
     def _states(self, states, n):

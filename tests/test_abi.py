@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), n
         assert n in _abi.SIGNATURES, n
-    assert L.pmo_abi_version() == _abi.ABI_VERSION == 10
+    assert L.pmo_abi_version() == _abi.ABI_VERSION == 11
     assert L.pmo_workspace_bytes() >= 8
 
 
@@ -39,7 +39,7 @@ PROBE = r"""
 int main(void) {
   S(pmo_volume) S(pmo_source) S(pmo_run) S(pmo_tallies) S(pmo_plate_instance) S(pmo_batch_tallies) S(pmo_diff) S(pmo_obd)
   O(pmo_diff, mismatch) O(pmo_plate_instance, n_packets) O(pmo_plate_instance, layers) O(pmo_run, key_dev) O(pmo_run, quorum)
-  O(pmo_volume, optics) O(pmo_volume, skip) O(pmo_source, w0) O(pmo_source, add) O(pmo_source, states_stride)
+  O(pmo_volume, optics) O(pmo_volume, skip) O(pmo_volume, label_radius) O(pmo_source, w0) O(pmo_source, add) O(pmo_source, states_stride)
   O(pmo_run, packets_per_lane) O(pmo_tallies, exit_state)
   O(pmo_tallies, ring_w) O(pmo_tallies, line_z) O(pmo_tallies, fluence_rz) O(pmo_run, flags) O(pmo_run, window)
   O(pmo_tallies, exit_add) O(pmo_tallies, exit_p) O(pmo_tallies, absorbed_lines) O(pmo_run, refill)
@@ -69,6 +69,7 @@ def test_ctypes_layout_matches_header(tmp_path):
         assert int(out[name]) == C.sizeof(cls), name
     assert int(out["pmo_volume.optics"]) == _abi.PmoVolume.optics.offset
     assert int(out["pmo_volume.skip"]) == _abi.PmoVolume.skip.offset
+    assert int(out["pmo_volume.label_radius"]) == _abi.PmoVolume.label_radius.offset
     assert int(out["pmo_source.w0"]) == _abi.PmoSource.w0.offset
     assert int(out["pmo_source.add"]) == _abi.PmoSource.add.offset
     assert int(out["pmo_source.states_stride"]) == _abi.PmoSource.states_stride.offset
