@@ -365,7 +365,38 @@ def workload_leg(wl, peaks, n_packets=100_000_000, walker=None, absorbed=True):
     if wl.tallies.get("rings"):
         out["rings_mm"] = [list(r) for r in wl.tallies["rings"]]
         out["ring_weight_per_packet"] = [float(x) for x in tal.hist["ring_w"].cpu().numpy() / n_packets]
+    if absorbed and glob_pp > 0:
+        # the atomic rate at this map's footprint: above the 256 MB of Infinity Cache the
+        # device's FP64 atomics run slower than the 32 MiB probe of the headline roofline
+        del tal
+        torch.cuda.empty_cache()
+        peak = atomic_peak_at(wk.nvox)
+        out["atomic_peak_at_map_footprint"] = peak
+        out["atomic_frac_at_map_footprint"] = glob_pp * n_packets / t / peak
     return out
+
+
+def atomic_peak_at(n_doubles, blocks=4096, iters=256):
+    """FP64 atomic adds per second (libpmo's probe, no return value) on pseudo-random
+    addresses of a buffer of n_doubles, the best of two timed runs."""
+    import ctypes as C
+    import torch
+    from pymonteopt_amd import _abi
+    L = _abi.lib()
+    n = int(min(max(n_doubles, 1), (1 << 32) - 1))
+    buf = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    best = 0.0
+    for _ in range(2):
+        e0, e1 = _events()
+        e0.record()
+        _abi.check(L.pmo_probe_atomics(C.c_void_p(buf.data_ptr()), n, blocks, iters, _abi.PMO_PROBE_F64, s))
+        e1.record()
+        e1.synchronize()
+        best = max(best, blocks * 256 * iters / (e0.elapsed_time(e1) * 1e-3))
+    del buf
+    torch.cuda.empty_cache()
+    return best
 
 
 def slab_leg(n_batches=10, n_packets=100_000):

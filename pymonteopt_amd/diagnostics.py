@@ -45,7 +45,8 @@ def peaks():
     count = blocks * 256 * iters
     kinds = (("f64", _abi.PMO_PROBE_F64, 8), ("f32", _abi.PMO_PROBE_F32, 4), ("u64", _abi.PMO_PROBE_U64, 8),
              ("u32", _abi.PMO_PROBE_U32, 4))
-    for mib in (0.25, 2, 32, 256):
+    # footprints past the 256 MB Infinity Cache (MALL) as well: the device's atomics slow there
+    for mib in (0.25, 2, 32, 256, 512, 1024, 2048, 4096):
         for name, kind, size in kinds:
             if name != "f64" and mib not in (2, 32):
                 continue
