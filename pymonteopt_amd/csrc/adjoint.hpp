@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
                         inter = 0; fres = 0; nev = 0;
                         L0 = L1 = L2 = L3 = N0 = N1 = N2 = N3 = (R)0;
                         pq = philox_packet(kTagWalk, plo, P.seed);
-                        const uint4 r = philox_packet_block(pq, 0u, phi, P.seed, P.stream);
+                        const uint4 r = philox_packet_block_sk(pq, 0u, phi, P.seed, P.stream);
                         st = M::neglog(u01<R>(r.x));
                         u_roul = u01<R>(r.y); u_th = u01<R>(r.z); u_phi = u01<R>(r.w);
                         alive = true;
@@ -335,7 +335,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
                 if (ni != nt) {
                     R u;
                     if ((fres & 3u) == 0u) {
-                        const uint4 r = philox4x32_10(make_uint4(fres >> 2, kTagFresnel, (uint32_t)pid,
+                        const uint4 r = philox4x32_10_sk(make_uint4(fres >> 2, kTagFresnel, (uint32_t)pid,
                                                                  (uint32_t)(pid >> 32)), P.seed, P.stream);
                         u = u01<R>(r.x); fb0 = u01<R>(r.y); fb1 = u01<R>(r.z); fb2 = u01<R>(r.w);
                     } else {
@@ -398,7 +398,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
                     spin<R>(hg_cos<R>(tab[lab], u_th), u_phi, vx, vy, vz);
                     ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
                     ++inter;
-                    const uint4 r = philox_packet_block(pq, inter, (uint32_t)(pid >> 32), P.seed, P.stream);
+                    const uint4 r = philox_packet_block_sk(pq, inter, (uint32_t)(pid >> 32), P.seed, P.stream);
                     st = M::neglog(u01<R>(r.x));
                     u_roul = u01<R>(r.y); u_th = u01<R>(r.z); u_phi = u01<R>(r.w);
                 } else {

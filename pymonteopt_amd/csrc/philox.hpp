@@ -36,6 +36,25 @@ __host__ __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, ui
     return c;
 }
 
+// The same block for a key held in scalar registers (uniform across the wave): the round
+// keys are stepped in place, one scalar add each, so the compiler cannot precompute all
+// twenty of them into registers that, in a kernel short of scalar registers, it would spill
+// to vector-register lanes and read back with a VALU v_readlane per round.
+__device__ __forceinline__ uint4 philox4x32_10_sk(uint4 c, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t m0 = (uint64_t)kPhiloxM0 * c.x;
+        const uint64_t m1 = (uint64_t)kPhiloxM1 * c.z;
+        c = make_uint4((uint32_t)(m1 >> 32) ^ c.y ^ k0, (uint32_t)m1,
+                       (uint32_t)(m0 >> 32) ^ c.w ^ k1, (uint32_t)m0);
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
+        asm volatile("" : "+s"(k0), "+s"(k1));
+    }
+    return c;
+}
+
 // A packet's blocks of one tag, counter (block, tag, plo, phi): round 1's x and y words and
 // round 2's product M0 * x depend only on the tag, plo and the key, not on the block index.
 // philox_packet computes them once per packet; philox_packet_block then gives block `blk`
@@ -80,6 +99,35 @@ __host__ __device__ __forceinline__ uint4 philox_packet_block(const PhiloxPacket
                        (uint32_t)(m0 >> 32) ^ c.w ^ k1, (uint32_t)m0);
         k0 += kPhiloxW0;
         k1 += kPhiloxW1;
+    }
+    return c;
+}
+
+// philox_packet_block for keys held in scalar registers, round keys stepped in place as in
+// philox4x32_10_sk
+__device__ __forceinline__ uint4 philox_packet_block_sk(const PhiloxPacket& q, uint32_t blk, uint32_t phi,
+                                                        uint32_t k0, uint32_t k1)
+{
+    const uint64_t a = (uint64_t)kPhiloxM0 * blk;
+    const uint32_t z1 = (uint32_t)(a >> 32) ^ phi ^ k1;
+    const uint32_t w1 = (uint32_t)a;
+    k0 += kPhiloxW0;
+    k1 += kPhiloxW1;
+    asm volatile("" : "+s"(k0), "+s"(k1));
+    const uint64_t b = (uint64_t)kPhiloxM1 * z1;
+    uint4 c = make_uint4((uint32_t)(b >> 32) ^ q.y1 ^ k0, (uint32_t)b, q.h2 ^ w1 ^ k1, q.w2);
+    k0 += kPhiloxW0;
+    k1 += kPhiloxW1;
+    asm volatile("" : "+s"(k0), "+s"(k1));
+#pragma unroll
+    for (int r = 2; r < 10; ++r) {
+        const uint64_t m0 = (uint64_t)kPhiloxM0 * c.x;
+        const uint64_t m1 = (uint64_t)kPhiloxM1 * c.z;
+        c = make_uint4((uint32_t)(m1 >> 32) ^ c.y ^ k0, (uint32_t)m1,
+                       (uint32_t)(m0 >> 32) ^ c.w ^ k1, (uint32_t)m0);
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
+        asm volatile("" : "+s"(k0), "+s"(k1));
     }
     return c;
 }

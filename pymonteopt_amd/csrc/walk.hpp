@@ -515,7 +515,7 @@ __device__ __forceinline__ void walk_body(const WalkParams& P)
                     ni = tab[lab].n;
                     mt = tab[lab].mt;
                     inter = 0; fres = 0; nev = 0;
-                    const uint4 r = philox4x32_10(make_uint4(0u, kTagWalk, plo, phi), seed, skey);
+                    const uint4 r = philox4x32_10_sk(make_uint4(0u, kTagWalk, plo, phi), seed, skey);
                     st = M::neglog(u01<R>(r.x));
                     u_roul = u01<R>(r.y); u_th = u01<R>(r.z); u_phi = u01<R>(r.w);
                     alive = true;
@@ -616,7 +616,7 @@ __device__ __forceinline__ void walk_body(const WalkParams& P)
                     // Fresnel (:82-108) in cosine form; equal to the acos/asin/sin/tan form
                     R u;
                     if ((fres & 3u) == 0u) {
-                        const uint4 r = philox4x32_10(make_uint4(fres >> 2, kTagFresnel, (uint32_t)pid,
+                        const uint4 r = philox4x32_10_sk(make_uint4(fres >> 2, kTagFresnel, (uint32_t)pid,
                                                                  (uint32_t)(pid >> 32)), seed, skey);
                         u = u01<R>(r.x); fb0 = u01<R>(r.y); fb1 = u01<R>(r.z); fb2 = u01<R>(r.w);
                     } else {
@@ -721,8 +721,8 @@ __device__ __forceinline__ void walk_body(const WalkParams& P)
                     spin<R>(hg_cos<R>(L, u_th), u_phi, vx, vy, vz);      // (:143-165)
                     ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
                     ++inter;
-                    const uint4 r = philox4x32_10(make_uint4(inter, kTagWalk, (uint32_t)pid,
-                                                             (uint32_t)(pid >> 32)), seed, skey);
+                    const uint4 r = philox4x32_10_sk(make_uint4(inter, kTagWalk, (uint32_t)pid,
+                                                                (uint32_t)(pid >> 32)), seed, skey);
                     st = M::neglog(u01<R>(r.x));
                     u_roul = u01<R>(r.y); u_th = u01<R>(r.z); u_phi = u01<R>(r.w);
                 } else {
