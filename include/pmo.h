@@ -107,15 +107,16 @@ typedef struct pmo_volume {
     const float* optics;   /* device, float4 per label, n_labels+1 entries: (mu_a, mu_s, g, n);
                               entry n_labels holds n_ambient (mu_a, mu_s, g unused)            */
     const uint8_t* skip;   /* device [nx][ny][nz], pmo_skip_radius of these labels, or NULL.
-                              With it pmo_walk crosses the faces inside same-label regions in
-                              one step (the same draws; merged arithmetic, rounded once); NULL
-                              steps voxel by voxel as the reference does. pmo_walk_forward_diff
-                              uses it without mua_field; other entry points ignore it.        */
-    const uint16_t* label_radius; /* device [nx][ny][nz], labels[v] | skip[v] << 8 of each voxel
-                              (pmo_pack_label_radius), or NULL. With skip, pmo_walk reads a
-                              voxel's label and radius from it in one 2-byte load (one memory
-                              access per crossing instead of two); it must hold exactly labels
-                              and skip. Other entry points ignore it. No reference counterpart. */
+                              pmo_walk_forward_diff (without mua_field) crosses the faces
+                              inside same-label regions in one step with it; pmo_walk reads the
+                              packed form below instead and rejects skip without it.         */
+    const uint16_t* label_radius; /* device [nx][ny][nz], labels[v] | radius[v] << 8
+                              (pmo_pack_label_radius of the labels and their pmo_skip_radius),
+                              or NULL. With it pmo_walk crosses the faces inside same-label
+                              regions in one step (the same draws; merged arithmetic, rounded
+                              once), reading a voxel's label and radius with one 2-byte load;
+                              NULL steps voxel by voxel as the reference does. It must hold
+                              exactly `labels`. No reference counterpart.                      */
 } pmo_volume;
 
 typedef struct pmo_source {

@@ -222,6 +222,9 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
         if (!(dn > 0.999f && dn < 1.001f)) return fail(PMO_ERR_INVALID, "launch direction is not a unit vector");
     }
     if (!(run->roulette_m >= 1.f)) return fail(PMO_ERR_INVALID, "roulette_m must be >= 1");
+    if (vol->skip && !vol->label_radius)
+        return fail(PMO_ERR_INVALID, "pmo_walk skips same-label faces from pmo_volume.label_radius: pack the labels "
+                                     "with their radius (pmo_pack_label_radius) or pass skip = NULL");
     if (run->n_packets == 0) return PMO_OK;
     const bool whole = nvox <= kWholeVolumeCells;
     const bool fp64 = (run->flags & PMO_F_FP64) != 0u;
@@ -247,8 +250,7 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
     pmo::WalkParams P;
     memset(&P, 0, sizeof(P));
     P.labels = vol->labels;
-    P.skip = vol->skip;
-    P.label_radius = vol->skip ? vol->label_radius : nullptr;
+    P.label_radius = vol->label_radius;
     P.optics = reinterpret_cast<const float4*>(vol->optics);
     P.nx = (uint32_t)vol->nx; P.ny = (uint32_t)vol->ny; P.nz = (uint32_t)vol->nz;
     P.nvox = (uint32_t)nvox;
@@ -349,7 +351,7 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
         // keep as many blocks per CU as the registers allow: each gets its share of the CU's
         // LDS, the line buffers first, the window from the rest
         int nb_reg = 0;
-        rc = blocks_per_cu<float>(bs, 0, &nb_reg, false, P.skip != nullptr, true);
+        rc = blocks_per_cu<float>(bs, 0, &nb_reg, false, P.label_radius != nullptr, true);
         if (rc != PMO_OK) return rc;
         const size_t share = d.lds_per_cu / (size_t)(nb_reg > 0 ? nb_reg : 1);
         const size_t cells = share > P.lds_win ? (share - P.lds_win) / sizeof(double) : 0;
@@ -381,7 +383,7 @@ int pmo_walk(const pmo_volume* vol, const pmo_source* src, const pmo_run* run, c
         return fail(PMO_ERR_UNSUPPORTED, "histograms and tally window need %zu B of LDS (limit %zu)", lds, d.max_lds);
 
     int nb = 0;
-    const bool skip = P.skip != nullptr;
+    const bool skip = P.label_radius != nullptr;
     rc = fp64 ? blocks_per_cu<double>(bs, lds, &nb, P.fluence_rz != nullptr, skip)
               : blocks_per_cu<float>(bs, lds, &nb, P.fluence_rz != nullptr, skip, lines);
     if (rc != PMO_OK) return rc;

@@ -82,8 +82,7 @@ template <typename R> struct LabelConst {
 // ------------------------------------------------------------------ kernel parameters
 struct WalkParams {
     const uint8_t* labels;
-    const uint8_t* skip;       // uniform-cube radius per voxel (skip_*_kernel), or null
-    const uint16_t* label_radius;  // label | radius << 8 per voxel (with skip), or null
+    const uint16_t* label_radius;  // label | radius << 8 per voxel (SKIP kernels), or null
     const float4* optics;
     uint32_t nx, ny, nz, nvox;
     uint32_t sx, sy;           // strides of x and y in the C-order volume
@@ -380,7 +379,8 @@ __device__ __forceinline__ void flush_lines(double* lines, uint32_t stride, doub
 
 // ------------------------------------------------------------------ the walk kernel
 // RZ: also tally the r-z fluence (a separate instantiation, so the plain walk keeps its
-// register budget). SKIP: P.skip holds each voxel's uniform-cube radius, and a packet in a
+// register budget). SKIP: P.label_radius holds each voxel's label and uniform-cube radius
+// (label | radius << 8, read with one load), and a packet in a
 // voxel of radius r > 0 treats the cube of half-width r around it as one voxel: one step
 // reaches its face or the hop's end. The faces inside the cube join voxels of one label,
 // where the reference moves on without a draw (_kernel.py:82), so every packet makes the
@@ -502,13 +502,12 @@ __device__ __forceinline__ void walk_body(const WalkParams& P)
                     rad = 0;
                     if ((uint32_t)ax < P.nx && (uint32_t)ay < P.ny && (uint32_t)az < P.nz) {
                         const uint32_t li = ((uint32_t)ax * P.ny + (uint32_t)ay) * P.nz + (uint32_t)az;
-                        if (SKIP && P.label_radius) {
+                        if constexpr (SKIP) {
                             const uint32_t lr = P.label_radius[li];
                             l0 = lr & 255u;
                             rad = (int32_t)(lr >> 8);
                         } else {
                             l0 = P.labels[li];
-                            if (SKIP) rad = P.skip[li];
                         }
                     }
                     lab = l0 < end ? l0 : end;
@@ -599,14 +598,13 @@ __device__ __forceinline__ void walk_body(const WalkParams& P)
                 int32_t nrad = 0;
                 if ((uint32_t)bx < P.nx && (uint32_t)by < P.ny && (uint32_t)bz < P.nz) {
                     const uint32_t nvi = (__umul24((uint32_t)bx, P.ny) + (uint32_t)by) * P.nz + (uint32_t)bz;
-                    if (SKIP && P.label_radius) {
+                    if constexpr (SKIP) {
                         // label and radius in one 2-byte load: one memory access per crossing
                         const uint32_t lr = P.label_radius[nvi];
                         nxt = lr & 255u;
                         nrad = (int32_t)(lr >> 8);
                     } else {
                         nxt = P.labels[nvi];
-                        if (SKIP) nrad = P.skip[nvi];
                     }
                     nxt = nxt < end ? nxt : end;
                 }

@@ -116,6 +116,7 @@ def _valid_structs():
     ("ix", 0, "launch voxel"), ("kind", 9, "beam kind"), ("kind", 3, "per-packet launch states"),
     ("block_size", 100, "block_size"),
     ("roulette_m", 0.5, "roulette_m"), ("scalars", None, "scalars"), ("nx", 2, "extents"),
+    ("skip", 1, "label_radius"),
 ])
 def test_argument_errors_are_codes(field, value, msg):
     # every check below runs on the host before any HIP call
@@ -129,6 +130,14 @@ def test_argument_errors_are_codes(field, value, msg):
     assert msg in L.pmo_last_error().decode()
     with pytest.raises(ValueError):
         _abi.check(rc)
+
+
+def test_pack_label_radius_argument_errors():
+    # checked on the host before any HIP call
+    L = _abi.lib()
+    assert L.pmo_pack_label_radius(None, C.c_void_p(1), C.c_void_p(1), 5, None) == _abi.PMO_ERR_INVALID
+    assert L.pmo_pack_label_radius(C.c_void_p(1), C.c_void_p(1), C.c_void_p(1), -1, None) == _abi.PMO_ERR_INVALID
+    assert L.pmo_pack_label_radius(None, None, None, 0, None) == _abi.PMO_OK
 
 
 @pytest.mark.parametrize("args,msg", [
