@@ -353,7 +353,13 @@ plate_batch_kernel(const BatchParams P)
                         ivx = M::rcp(M::fabs(vx)); ivy = M::rcp(M::fabs(vy)); ivz = M::rcp(M::fabs(vz));
                     }
                     ++inter;
-                    const uint4 r = philox_packet_block(pq, inter, 0u, seed, key1);
+                    // the instance key is per lane; when every lane in this step has the
+                    // same one (a wave mostly walks one batch of one instance), its round
+                    // keys step in scalar registers instead of one vector add per round
+                    const uint32_t k1u = __builtin_amdgcn_readfirstlane(key1);
+                    uint4 r;
+                    if (__ballot(key1 != k1u) == 0ull) r = philox_packet_block_sk(pq, inter, 0u, seed, k1u);
+                    else r = philox_packet_block(pq, inter, 0u, seed, key1);
                     st = M::neglog(u01<R>(r.x));
                     u_roul = u01<R>(r.y); u_th = u01<R>(r.z); u_phi = u01<R>(r.w);
                 } else {
