@@ -153,8 +153,9 @@ def test_skin_pencil_absorbed_map_same_seeds(torch_cuda, oracle):
 
 def test_skin_pencil_absorbed_map_wide_launch(torch_cuda, oracle):
     """The bench workload with the launch the bench uses (>= 2^20 packets per walk: 1024-lane
-    blocks, 28x28x24 LDS window): GPU seeds 500-509 x 1.2e6 against oracle seeds 100-109 x
-    1e5, per packet; scalars, Rd(r), the depth profile and the (r, z) projection of the map."""
+    blocks, a 22x22x22 LDS window, cube line buffers): GPU seeds 500-509 x 1.2e6 against
+    oracle seeds 100-109 x 1e5, per packet; scalars, Rd(r), the depth profile and the (r, z)
+    projection of the map."""
     from pymonteopt_amd import presets
     from pymonteopt_amd.runtime import Walker
     wl = presets.skin_plate()
@@ -177,6 +178,31 @@ def test_skin_pencil_absorbed_map_wide_launch(torch_cuda, oracle):
     for g in G:
         assert ledger_error(g["scalars"]) < 1e-6
         assert g["events"]["global_deposits"] < g["events"]["interactions"]   # the LDS window took some
+
+
+def test_skin_pencil_at_the_bench_run_size(torch_cuda, oracle):
+    """The bench's own run: the skin plate, pencil beam, absorbed map, 1e8 packets per walk
+    (GPU seeds 900-902), against the oracle on independent seeds (10 x 4e5); per packet,
+    Rd, Tt and A within 3 pooled standard errors, Rd(r) binned, the ledger of each walk."""
+    from pymonteopt_amd import presets
+    from pymonteopt_amd.runtime import Walker
+    wl = presets.skin_plate()
+    wk = Walker(wl.labels, wl.pitch, wl.optics)
+    NG, NO = 100_000_000, 400_000
+    G = []
+    for s in range(3):
+        print(f"skin pencil + map, {NG} packets, seed {900 + s}", flush=True)
+        G.append(gpu_run(wk, wl, NG, 900 + s))
+    print(f"oracle {NO} x 10", flush=True)
+    O = [oracle_run(oracle, wl, NO, 200 + s) for s in range(10)]
+    for k in ("Rd", "Tt", "A"):
+        assert zscore(_per_packet(G, k, NG), _per_packet(O, k, NO)) < 3, k
+    ok, info = _bins_ok(binned_z([g["rd_r"] / NG for g in G], [o["rd_r"] / NO for o in O]))
+    assert ok, ("rd_r", info)
+    for g in G:
+        assert g["events"]["packets"] == NG
+        assert ledger_error(g["scalars"]) < 1e-6
+        assert g["absorbed"].sum() == pytest.approx(g["scalars"]["A"], rel=1e-9)
 
 
 # ------------------------------------------------------------------------- exit tallies
