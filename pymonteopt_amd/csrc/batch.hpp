@@ -7,7 +7,12 @@
 //
 // Work is a schedule of packet batches (instance, first local id, count), ordered
 // round-robin over instances so that every instance's tallies are updated throughout the
-// launch instead of in one burst. A wave claims one batch per counter atomic. Packet
+// launch instead of in one burst. A wave claims one batch per counter atomic and sums the
+// tallies of that batch's instance in its own LDS block (the instance's scalars, then its
+// Rd(r) and Tt(r) bins), which it adds to global memory when it claims a batch of another
+// instance: a few dozen global atomics per batch instead of three per packet, which
+// otherwise contend on each instance's few lines. A packet of an earlier instance, still
+// walking when its wave moves on, adds its tallies to global memory directly. Packet
 // (instance i, local id j) draws from Philox key (seed, stream + i) with packet id j, the
 // same draws as a plain walk of instance i alone with stream + i, so each instance's
 // result is that of its own plain walk.
@@ -48,6 +53,7 @@ struct BatchParams {
     double* tt_r;               // [n_inst][nr]
     unsigned long long* work;
     const uint32_t* key_dev;
+    uint32_t wave_acc;          // doubles of a wave's LDS tally of its current instance (0: none)
 };
 
 // the fields of the current instance a lane needs on every event, kept in registers
@@ -88,7 +94,11 @@ plate_batch_kernel(const BatchParams P)
 {
     using M = Mth<R>;
     __shared__ unsigned long long bev[kBE_COUNT];
+    extern __shared__ __align__(16) unsigned char smem_b[];
     for (uint32_t i = threadIdx.x; i < kBE_COUNT; i += blockDim.x) bev[i] = 0ull;
+    // the wave's tally of instance w_inst: [kBS_COUNT scalars | nr Rd(r) | nr Tt(r)]
+    double* wacc = reinterpret_cast<double*>(smem_b) + (threadIdx.x >> 6) * P.wave_acc;
+    for (uint32_t i = threadIdx.x & 63u; i < P.wave_acc; i += 64u) wacc[i] = 0.0;
     __syncthreads();
 
     const uint32_t lane = threadIdx.x & 63u;
@@ -114,6 +124,37 @@ plate_batch_kernel(const BatchParams P)
     // wave-uniform current batch
     uint32_t b_inst = 0, b_next = 0, b_end = 0;
     bool exhausted = false;
+    int32_t w_inst = -1;              // wave-uniform: the instance the wave's LDS tally holds
+
+    // a tally of the lane's instance: into the wave's LDS block when it holds that instance
+    auto inst_add = [&](uint32_t j, double v) {
+        if (inst == w_inst && P.wave_acc != 0u) lds_add(&wacc[j], v);
+        else if (j < (uint32_t)kBS_COUNT) gmem_add(P.scalars + (size_t)inst * kBS_COUNT + j, v);
+        else if (j < (uint32_t)kBS_COUNT + (uint32_t)P.nr) gmem_add(P.rd_r + (size_t)inst * P.nr + (j - kBS_COUNT), v);
+        else gmem_add(P.tt_r + (size_t)inst * P.nr + (j - kBS_COUNT - (uint32_t)P.nr), v);
+    };
+    // every lane of the wave together: the LDS block to global memory, cleared
+    auto flush_wave = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (w_inst >= 0) {
+            for (uint32_t j = lane; j < P.wave_acc; j += 64u) {
+                const double v = wacc[j];
+                if (v != 0.0) {
+                    wacc[j] = 0.0;
+                    double* dst = j < (uint32_t)kBS_COUNT ? P.scalars + (size_t)w_inst * kBS_COUNT + j
+                                : (j < (uint32_t)kBS_COUNT + (uint32_t)P.nr
+                                       ? P.rd_r + (size_t)w_inst * P.nr + (j - kBS_COUNT)
+                                       : P.tt_r + (size_t)w_inst * P.nr + (j - kBS_COUNT - (uint32_t)P.nr));
+                    gmem_add(dst, v);
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
 
     for (;;) {
         const unsigned long long need = __ballot(!alive);
@@ -129,6 +170,10 @@ plate_batch_kernel(const BatchParams P)
                     b_inst = b.x < (uint32_t)P.n_inst ? b.x : 0u;
                     b_next = b.y;
                     b_end = b.x < (uint32_t)P.n_inst ? b.y + b.z : b.y;
+                    if (P.wave_acc != 0u && (int32_t)b_inst != w_inst) {
+                        flush_wave();
+                        w_inst = (int32_t)b_inst;
+                    }
                 }
             }
             if (!exhausted) {
@@ -342,7 +387,7 @@ plate_batch_kernel(const BatchParams P)
                     } else {
                         const R wm = w * (R)P.roulette_m;
                         const double gain = (double)wm - (double)w;
-                        gmem_add(P.scalars + (size_t)inst * kBS_COUNT + kBS_GAIN, gain);
+                        inst_add(kBS_GAIN, gain);
                         acc_abs += gain;
                         w = wm;
                     }
@@ -374,16 +419,15 @@ plate_batch_kernel(const BatchParams P)
             atomicAdd(&bev[kBE_INTERACT], (unsigned long long)inter);
             atomicAdd(&bev[kBE_FRESNEL], (unsigned long long)fres);
             atomicAdd(&bev[kBE_PACKETS], 1ull);
-            double* sc = P.scalars + (size_t)inst * kBS_COUNT;
             if (killed) {
                 atomicAdd(&bev[kBE_KILLED], 1ull);
-                gmem_add(sc + kBS_KILL, (double)w);
+                inst_add(kBS_KILL, (double)w);
             } else {
                 atomicAdd(&bev[kBE_ESCAPED], 1ull);
                 const double wd = (double)w;
                 const float fvz = (float)vz;
-                gmem_add(sc + (fvz < 0.f ? kBS_RD : (fvz > 0.f ? kBS_TT : kBS_OTHER)), wd);
-                gmem_add(sc + kBS_FACE + 2 * xk + (xs > 0 ? 1 : 0), wd);
+                inst_add(fvz < 0.f ? kBS_RD : (fvz > 0.f ? kBS_TT : kBS_OTHER), wd);
+                inst_add(kBS_FACE + 2 * xk + (xs > 0 ? 1 : 0), wd);
                 if (P.nr > 0 && fvz != 0.f) {
                     float x, y;
                     if constexpr (LAYER) {
@@ -395,10 +439,8 @@ plate_batch_kernel(const BatchParams P)
                         y = pitch * (float)(ay - cy) + (float)py;
                     }
                     const int b = bin_of(__builtin_sqrtf(x * x + y * y), P.dr, P.nr);
-                    if (b >= 0) {
-                        double* h = fvz < 0.f ? P.rd_r : P.tt_r;
-                        if (h) gmem_add(h + (size_t)inst * P.nr + b, wd);
-                    }
+                    if (b >= 0 && (fvz < 0.f ? P.rd_r : P.tt_r) != nullptr)
+                        inst_add((uint32_t)kBS_COUNT + (fvz < 0.f ? 0u : (uint32_t)P.nr) + (uint32_t)b, wd);
                 }
             }
             alive = false;
@@ -408,6 +450,7 @@ plate_batch_kernel(const BatchParams P)
         gmem_add(P.scalars + (size_t)inst * kBS_COUNT + kBS_ABS, acc_abs);
         gmem_add(P.scalars + (size_t)inst * kBS_COUNT + kBS_W0, acc_w0);
     }
+    if (P.wave_acc != 0u) flush_wave();
     atomicAdd(&bev[kBE_CROSS], (unsigned long long)n_cross);
     __syncthreads();
     if (P.events)

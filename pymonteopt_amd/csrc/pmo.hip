@@ -73,6 +73,10 @@ constexpr int kLinesQuorum = 40;
 // 32 loses 30%), the skin plate without the absorbed map +1%, with it and the bone phantom
 // unchanged.
 constexpr int kDefaultRefill = 4;
+// LDS a batched-walk block may spend on its waves' tallies of their current instances
+// (kBS_COUNT + 2 nr doubles per wave): 93 doubles per wave with 40 radial bins, 11.9 KiB for
+// a 1024-lane block. Past this (long Rd(r) tables) the waves add to global memory directly.
+constexpr size_t kWaveAccBytes = 32768;
 
 struct DevInfo {
     int n_cu = 0;
@@ -854,11 +858,15 @@ int pmo_walk_plates(const pmo_plate_instance* inst, int32_t n_inst, const uint32
     P.rd_r = tal->nr > 0 ? tal->rd_r : nullptr;
     P.tt_r = tal->nr > 0 ? tal->tt_r : nullptr;
     P.work = reinterpret_cast<unsigned long long*>(workspace);
+    // each wave's LDS tally of its current instance, while it stays small beside the walk
+    const size_t acc = (size_t)pmo::kBS_COUNT + 2u * (size_t)(tal->nr > 0 ? tal->nr : 0);
+    P.wave_acc = acc * sizeof(double) * (size_t)(bs / 64) <= kWaveAccBytes ? (uint32_t)acc : 0u;
+    const size_t lds = (size_t)P.wave_acc * sizeof(double) * (size_t)(bs / 64);
 
     const bool layer = (run->flags & PMO_F_LAYER_STEP) != 0u;
     int nb = 0;
-    hipError_t e = layer ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::plate_batch_kernel<float, true>, bs, 0)
-                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::plate_batch_kernel<float, false>, bs, 0);
+    hipError_t e = layer ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::plate_batch_kernel<float, true>, bs, lds)
+                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pmo::plate_batch_kernel<float, false>, bs, lds);
     if (e != hipSuccess) return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     if (nb < 1) nb = 1;
     unsigned long long grid = run->grid_blocks > 0 ? (unsigned long long)run->grid_blocks
@@ -871,9 +879,9 @@ int pmo_walk_plates(const pmo_plate_instance* inst, int32_t n_inst, const uint32
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "zero workspace launch");
     if (layer)
-        hipLaunchKernelGGL((pmo::plate_batch_kernel<float, true>), dim3((unsigned)grid), dim3(bs), 0, s, P);
+        hipLaunchKernelGGL((pmo::plate_batch_kernel<float, true>), dim3((unsigned)grid), dim3(bs), lds, s, P);
     else
-        hipLaunchKernelGGL((pmo::plate_batch_kernel<float, false>), dim3((unsigned)grid), dim3(bs), 0, s, P);
+        hipLaunchKernelGGL((pmo::plate_batch_kernel<float, false>), dim3((unsigned)grid), dim3(bs), lds, s, P);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "batched walk launch");
     return PMO_OK;

@@ -80,3 +80,34 @@ def test_graph_replay_matches_eager():
         x = torch.ones(1000, device="cuda:0").sum()
         torch.cuda.synchronize()
         assert float(x) == 1000.0
+
+
+@pytest.mark.parametrize("layer_step", [False, True])
+def test_wave_lds_tallies_match_direct_adds(layer_step):
+    """Each wave sums its current plate's tallies in LDS while they fit (13 + 2 nr doubles a
+    wave, at most 32 KiB a block) and adds them to global memory otherwise. With 400 radial
+    bins a 256-lane block takes the LDS path (26 KiB) and a 1024-lane block the direct one
+    (104 KiB); the packets and their draws are the same, so every instance's sums agree to
+    the rounding of their order. Also the case the LDS path is for: waves that all walk the
+    same two plates."""
+    import torch
+    from pymonteopt_amd.batched import Plate, PlateSweep
+    two = [Plate([1.0], [0.1], [10.0], [0.9], [1.37], voxel_space=0.05, x_size=2.55, y_size=2.55, n_packets=200000),
+           Plate([0.5], [1.0], [5.0], [0.7], [1.45], voxel_space=0.05, x_size=5.05, y_size=5.05, n_packets=150000)]
+    for ps in (plates(), two):
+        got = {}
+        for bs in (256, 1024):
+            sw = PlateSweep(ps, nr=400, dr=0.01, batch=64, layer_step=layer_step)
+            sw.block_size = bs
+            print("plates", len(ps), "layer_step", layer_step, "block", bs, flush=True)
+            sw.run(seed=7, stream_id=3)
+            torch.cuda.synchronize()
+            r = sw.results()
+            assert r["events"]["packets"] == sum(p.n_packets for p in ps)
+            got[bs] = (r["scalars"].copy(), r["rd_r_raw"].copy(), sw.tt_r.cpu().numpy().copy(), r["events"])
+        (s1, rd1, tt1, e1), (s2, rd2, tt2, e2) = got[256], got[1024]
+        assert e1 == e2
+        assert np.allclose(s1, s2, rtol=1e-11, atol=1e-9)
+        assert np.allclose(rd1, rd2, rtol=1e-11, atol=1e-9)
+        assert np.allclose(tt1, tt2, rtol=1e-11, atol=1e-9)
+        assert rd1.sum() > 0 and tt1.sum() > 0
