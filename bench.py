@@ -183,8 +183,9 @@ def adjoint_leg(n_packets=10_000_000):
     """Gradients of the differentiable walk at 1e7 packets against the plain walk of the
     same packets (as the plain walk defaults, and voxel by voxel, which the replay follows):
     * skin plate, per-label d Rd(r) / d(mu_a, mu_s) as rd_of_label_optics computes it: one
-      forward with Jacobians (the backward is J^T g). The per-voxel replay of the same
-      plate is reported beside it;
+      forward with Jacobians (the backward is J^T g). The per-voxel gradient of the same
+      plate (rd_of_absorption_field: a forward with the absorption field, then the replay)
+      is reported beside it;
     * 64^3 4-label grid, per-voxel d Rd(r) / d mu_a as rd_of_absorption_field computes it:
       a forward with the absorption field and the replay adjoint.
     The replay's deposits outside its LDS box go either through cube line buffers or one FP64
@@ -222,20 +223,25 @@ def adjoint_leg(n_packets=10_000_000):
         tal = wk.tallies(TallySpec(nr=40, dr=0.1, absorbed=False))
         _, t_plain = timed(lambda: wk.walk(wl.source, n_packets, 3, tal))
         _, t_plain_v = timed(lambda: wkv.walk(wl.source, n_packets, 3, tal))
+        mua = torch.tensor(wl.optics[:, 0], device="cuda:0")
+        field = mua[torch.as_tensor(np.minimum(wl.labels, wl.n_labels), device="cuda:0").long()].float()
         if name == "skin":
             # rd_of_label_optics: one forward with Jacobians, crossing same-label cubes in one
-            # step (no replay follows); the per-voxel replay needs a voxel-stepping forward
+            # step (no replay follows). The per-voxel gradient of the same plate, as
+            # rd_of_absorption_field computes it: a voxel-stepping forward with the absorption
+            # field and no Jacobians, then the replay
             _, t_fwd = timed(lambda: dw.forward(seed=3, jac=True, replay=False))
-            fwd, t_fwd_v = timed(lambda: dw.forward(seed=3, jac=True))
+            _, t_fwd_v = timed(lambda: dw.forward(seed=3, jac=True))
+            fwd, t_fwd_f = timed(lambda: dw.forward(seed=3, mua_field=field, jac=False))
             _, t_adj = timed(lambda: dw.adjoint(fwd, dw.area.clone()))
             t_grad = t_fwd
             rec = {"gradient": "per-label mu_a, mu_s: forward with Jacobians", "forward_with_jacobian_s": t_fwd,
                    "forward_with_jacobian_voxel_stepping_s": t_fwd_v,
-                   "per_voxel_replay_s": t_adj, "per_voxel_replay_modes_s": replay_modes(fwd),
-                   "per_voxel_gradient_cost_vs_plain": (t_fwd_v + t_adj) / t_plain}
+                   "per_voxel_forward_s": t_fwd_f, "per_voxel_replay_s": t_adj,
+                   "per_voxel_replay_modes_s": replay_modes(fwd),
+                   "per_voxel_gradient_cost_vs_plain": (t_fwd_f + t_adj) / t_plain,
+                   "per_voxel_gradient_cost_vs_its_forward": (t_fwd_f + t_adj) / t_fwd_f}
         else:
-            mua = torch.tensor(wl.optics[:, 0], device="cuda:0")
-            field = mua[torch.as_tensor(np.minimum(wl.labels, wl.n_labels), device="cuda:0").long()].float()
             fwd, t_fwd = timed(lambda: dw.forward(seed=3, mua_field=field, jac=False))
             _, t_adj = timed(lambda: dw.adjoint(fwd, dw.area.clone()))
             t_grad = t_fwd + t_adj

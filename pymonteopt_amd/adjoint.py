@@ -134,8 +134,12 @@ class DiffWalk:
             raise ValueError("this forward pass crossed same-label cubes in one step (forward(replay=False)); "
                              "run forward(replay=True) for a record the adjoint can replay")
         if self.lines == "auto":
+            # each mode once untimed (its kernel's first launch carries one-time costs that
+            # can exceed the difference between the modes), then once timed
             self._line_scratch()
             times = {}
+            for mode in (True, False):
+                self._replay(fwd, grad_rd_r, mode)
             for mode in (True, False):
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 e0.record()
