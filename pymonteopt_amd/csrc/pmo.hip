@@ -77,6 +77,8 @@ constexpr int kDefaultRefill = 4;
 // (kBS_COUNT + 2 nr doubles per wave): 93 doubles per wave with 40 radial bins, 11.9 KiB for
 // a 1024-lane block. Past this (long Rd(r) tables) the waves add to global memory directly.
 constexpr size_t kWaveAccBytes = 32768;
+// LDS of a fluence-binning block's copy of the bins (8192 FP64 bins)
+constexpr size_t kFluenceLdsBytes = 65536;
 
 struct DevInfo {
     int n_cu = 0;
@@ -1053,11 +1055,18 @@ int pmo_fluence_bin(const float* p, int64_t stride, const float* w, int64_t n, i
     DevInfo d;
     int rc = device_info(&d);
     if (rc != PMO_OK) return rc;
-    unsigned long long blocks = ((unsigned long long)n + 255) / 256;
-    if (blocks > (unsigned long long)d.n_cu * 8ull) blocks = (unsigned long long)d.n_cu * 8ull;
-    hipLaunchKernelGGL(pmo::fluence_bin_kernel, dim3((unsigned)blocks), dim3(256), 0,
+    // bins that fit a block's LDS are summed there first (one 1024-lane block per CU, so
+    // each bin takes at most one global add per CU); larger tables take one add per point
+    const unsigned long long nbins = dims == 2 ? (unsigned long long)nr * nz : 4ull * nr * nr * nz;
+    const bool in_lds = nbins * sizeof(double) <= kFluenceLdsBytes;
+    const unsigned bs = in_lds ? 1024u : 256u;
+    unsigned long long blocks = ((unsigned long long)n + bs - 1) / bs;
+    const unsigned long long cap = (unsigned long long)d.n_cu * (in_lds ? 1ull : 8ull);
+    if (blocks > cap) blocks = cap;
+    const uint32_t lds_bins = in_lds ? (uint32_t)nbins : 0u;
+    hipLaunchKernelGGL(pmo::fluence_bin_kernel, dim3((unsigned)blocks), dim3(bs), (size_t)lds_bins * sizeof(double),
                        reinterpret_cast<hipStream_t>(stream), p, (unsigned long long)stride, w, (unsigned long long)n,
-                       dims, nr, dr, nz, dz, out);
+                       dims, nr, dr, nz, dz, out, lds_bins);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PMO_OK : hip_fail(e, "fluence binning launch");
 }

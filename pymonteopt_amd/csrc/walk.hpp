@@ -886,11 +886,18 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) 
     walk_body<float, false, SKIP, true>(P);
 }
 
-// Fluence2D / Fluence3D binning of points (pymopt/fluence/intarnal_fluence.py)
-__global__ void __launch_bounds__(256) fluence_bin_kernel(const float* p, unsigned long long stride, const float* w,
-                                                          unsigned long long n, int dims, int nr, float dr, int nz,
-                                                          float dz, double* out)
+// Fluence2D / Fluence3D binning of points (pymopt/fluence/intarnal_fluence.py). With
+// lds_bins > 0 (all the bins) each block sums its points in an LDS copy of the bins and adds
+// the non-zero ones to `out` at its end: points crowd a few bins near the beam, whose global
+// FP64 atomics would otherwise serialise on their lines.
+__global__ void __launch_bounds__(1024) fluence_bin_kernel(const float* p, unsigned long long stride, const float* w,
+                                                           unsigned long long n, int dims, int nr, float dr, int nz,
+                                                           float dz, double* out, uint32_t lds_bins)
 {
+    extern __shared__ __align__(16) unsigned char smem_h[];
+    double* h = reinterpret_cast<double*>(smem_h);
+    for (uint32_t i = threadIdx.x; i < lds_bins; i += blockDim.x) h[i] = 0.0;
+    if (lds_bins) __syncthreads();
     const unsigned long long step = (unsigned long long)gridDim.x * blockDim.x;
     for (unsigned long long j = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += step) {
         const float x = p[j], y = p[stride + j], z = p[2 * stride + j];
@@ -906,7 +913,13 @@ __global__ void __launch_bounds__(256) fluence_bin_kernel(const float* p, unsign
             if (!(ix >= 0 && ix < 2 * nr && iy >= 0 && iy < 2 * nr)) continue;
             b = ((unsigned long long)ix * (2 * nr) + iy) * nz + iz;
         }
-        gmem_add(out + b, (double)w[j]);
+        if (lds_bins) lds_add(&h[b], (double)w[j]);
+        else gmem_add(out + b, (double)w[j]);
+    }
+    if (lds_bins) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < lds_bins; i += blockDim.x)
+            if (h[i] != 0.0) gmem_add(out + i, h[i]);
     }
 }
 
