@@ -125,3 +125,30 @@ def test_model_api_returns_normalised_rz_fluence():
     raw = m._raw["fluence_rz"]
     assert np.allclose(F, raw / (vol[:, None] * 100000))
     assert np.allclose(res["rz_r"], (i + 0.5) * 0.1) and res["rz_z"].size == 30
+
+
+def test_binning_at_the_lds_table_limit():
+    """Tables of up to 8192 bins are summed in an LDS copy per block, larger ones with one
+    global add per point: 128 x 64 = 8192 r-z bins and 8193 x 1 both match numpy. With
+    thousands of bins along r, float32 r / dr is off by up to ~5e-4 of a bin, so the points
+    keep 1e-2 of a bin from every edge."""
+    from pymonteopt_amd.fluence import Fluence2D
+    rng = np.random.default_rng(3)
+    for nr, nz, dr, dz in ((128, 64, 0.02, 0.05), (8193, 1, 0.0004, 5.0)):
+        lat = nr * dr * 0.8
+        p = np.stack([rng.uniform(-lat, lat, 200000), rng.uniform(-lat, lat, 200000),
+                      rng.uniform(-0.2, nz * dz * 1.1, 200000)]).astype(np.float32)
+        q = p.astype(np.float64)
+        ok = np.ones(p.shape[1], bool)
+        for v, d in ((np.hypot(q[0], q[1]), dr), (q[2], dz)):
+            fr = v / d - np.floor(v / d)
+            ok &= (fr > 1e-2) & (fr < 1 - 1e-2)
+        p, q = p[:, ok], q[:, ok]
+        w = rng.uniform(0.1, 1.0, p.shape[1]).astype(np.float32)
+        f = Fluence2D(nr=nr, nz=nz, dr=dr, dz=dz)
+        f.saveFluesnce(p, w)
+        want, _, _ = np.histogram2d(np.hypot(q[0], q[1]), q[2],
+                                    bins=[np.arange(nr + 1) * dr, np.arange(nz + 1) * dz], weights=w.astype(np.float64))
+        print("bins", nr * nz, "points", w.size, flush=True)
+        assert want.sum() > 0.5 * w.sum()
+        assert np.allclose(f.getArz(), want, rtol=1e-6, atol=1e-9)
